@@ -1,0 +1,135 @@
+/*
+ * kmc_dn_amd -- C-ABI of the MI355X KMC hop-loop engine (libkmc_dn_amd.so).
+ *
+ * Plain pointers and sizes only.  Every entry point returns an int status
+ * (KMC_OK == 0); on failure kmc_last_error() returns a thread-local message.
+ * Nothing here aborts the process.
+ *
+ * Reference interfaces replaced (MUTUEL/kmc_dn):
+ *   kmc_simulate_discrete   _simulate_discrete_record, kmc_dopant_networks.py:33-135
+ *                           (called from makeSimulation, :582, :598-611)
+ *   kmc_layout_* / kmc_eset_* / kmc_batch_*
+ *                           parallelSimulations (goSimulation/simulationWrapper.go:274-315)
+ *                           and channelSimulateRecord (:250-272): many independent
+ *                           networks in one call, here as one device-resident batch
+ *   wrapperSimulate, wrapperSimulateRecord, wrapperSimulateRecordPlus,
+ *   wrapperSimulatePruned, parallelSimulations
+ *                           exact-name exports with the cgo GoSlice signatures of
+ *                           simulationWrapper.go:83-169, :274-315, so the reference's
+ *                           ctypes callers (goSimulation/pythonBind.py:49-90,
+ *                           goSimulation/parrallelSimulationBind.py:50-79) can load
+ *                           this library unchanged.
+ */
+#ifndef KMC_DN_AMD_H
+#define KMC_DN_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KMC_NMAX 256 /* acceptors per network */
+#define KMC_PMAX 16  /* electrodes per network */
+#define KMC_OCCW 4   /* 64-bit words of an occupation bitmask */
+
+enum {
+  KMC_OK = 0,
+  KMC_ERR_ARGS = 1,        /* bad sizes / pointers / ids */
+  KMC_ERR_ZERO_RATE = 2,   /* total rate 0 or not finite (reference: ZeroDivisionError) */
+  KMC_ERR_NOMEM = 3,       /* device or host allocation failed */
+  KMC_ERR_DEVICE = 4,      /* HIP runtime error */
+  KMC_ERR_UNSUPPORTED = 5, /* entry point / mode outside this engine's scope */
+  KMC_ERR_LIMIT = 6        /* network larger than the kernel's limits */
+};
+
+/* Precision / kernel of a batch run. */
+enum {
+  KMC_FP64_DENSE = 0,  /* reference arithmetic: dense S^2 FP64 rates, full energy
+                          recompute, sequential row-major prefix sum */
+  KMC_FP32_BATCHED = 1 /* FP32 log-domain rates, O(N) rank-1 energy update with a
+                          periodic FP64 recompute, FP64 wave scan + binary pick */
+};
+
+const char* kmc_last_error(void);
+int kmc_version(void);
+
+/* Selects the HIP device and creates the engine stream (idempotent). */
+int kmc_init(int device);
+/* Name, CU count, LDS bytes per workgroup of the current device. */
+int kmc_device_info(char* name, int name_len, int* cu_count, int* lds_bytes);
+/* Frees every layout, energy set and batch. */
+int kmc_reset(void);
+
+/*
+ * _simulate_discrete_record (kmc_dopant_networks.py:33-135): `hops` hops of one
+ * network with the reference's FP64 dense arithmetic, one trajectory per launch.
+ * S = N + P.  distances, transitions_constant: S*S row-major; E_constant: N;
+ * site_energies: S ([N:] = electrode voltages; [0:N] overwritten with the last
+ * hop's energies); occupation: N bytes 0/1 (updated in place);
+ * electrode_occupation: P (updated in place).  record != 0 fills traffic (S*S,
+ * gross counts) and occupations_in_time (N, post-hop attribution), as the
+ * reference does.  uniforms: 2*hops replayed uniforms (clock, event per hop) or
+ * NULL for the Philox stream (seed, counter hop0 + h).  events_out / totals_out
+ * (optional, length hops) log the event index from*S+to and problist[-1].
+ */
+int kmc_simulate_discrete(int64_t N, int64_t P, double nu, double kT, double I_0, double R,
+                          const double* distances, const double* E_constant,
+                          double* site_energies, const double* transitions_constant,
+                          uint8_t* occupation, int64_t* electrode_occupation, int64_t hops,
+                          int record, double* traffic, double* occupations_in_time,
+                          const double* uniforms, uint64_t seed, uint64_t hop0,
+                          int32_t* events_out, double* totals_out, double* time_out);
+
+/* ---- device-resident batches (parallelSimulations, simulationWrapper.go:274-315) ---- */
+
+/* Uploads one network geometry.  distances/transitions_constant: S*S row-major.
+ * prune_threshold > 0 drops pairs with tc <= prune_threshold * max(tc) from the
+ * FP32 kernel's pair set (wrapperSimulatePruned, simulation.go:199-215). */
+int kmc_layout_create(int64_t N, int64_t P, double nu, double kT, double I_0, double R,
+                      const double* distances, const double* transitions_constant,
+                      double prune_threshold, int32_t* layout_out);
+
+/* Uploads `count` energy sets of a layout: E_constant count*N, V count*P
+ * (electrode voltages, i.e. site_energies[N:]).  Ids first..first+count-1. */
+int kmc_eset_create(int32_t layout, int64_t count, const double* E_constant, const double* V,
+                    int32_t* first_out);
+
+typedef struct kmc_traj {
+  int32_t layout;             /* from kmc_layout_create */
+  int32_t eset;               /* from kmc_eset_create (same layout) */
+  uint64_t seed;              /* Philox key */
+  uint64_t hop0;              /* Philox counter of the first hop */
+  int64_t prehops;            /* equilibration hops, not tallied */
+  int64_t hops;               /* tallied hops */
+  uint64_t occ[KMC_OCCW];     /* initial occupation bitmask, bit i = acceptor i */
+} kmc_traj;
+
+typedef struct kmc_run_opts {
+  int32_t precision;          /* KMC_FP64_DENSE or KMC_FP32_BATCHED */
+  int32_t waves_per_group;    /* FP32: trajectories per workgroup (1,2,4,8); 0 = auto */
+  int32_t recompute_interval; /* FP32: hops between FP64 energy recomputes; 0 = 1024 */
+  int32_t record;             /* 1: accumulate per-site occupation time */
+  int32_t post_hop_record;    /* 1: numba convention (post-hop state), 0: pre-hop */
+  int32_t grid_limit;         /* FP32: max resident workgroups; 0 = auto */
+  int32_t use_graph;          /* 1: launch through a captured hipGraph */
+  int32_t pad;
+} kmc_run_opts;
+
+/* Uploads trajectory descriptors and allocates device outputs. */
+int kmc_batch_create(const kmc_traj* trajs, int64_t ntraj, int32_t* batch_out);
+/* Enqueues the batch on the engine stream (asynchronous). */
+int kmc_batch_run(int32_t batch, const kmc_run_opts* opts);
+/* Waits for the last run; elapsed_ms (optional) = HIP-event time of its kernels. */
+int kmc_batch_sync(int32_t batch, double* elapsed_ms);
+/* Copies results to host arrays (each may be NULL):
+ * time[ntraj], eo[ntraj*KMC_PMAX], occ[ntraj*KMC_OCCW], status[ntraj],
+ * hops_done[ntraj], occ_time[ntraj*KMC_NMAX] (only after a record run). */
+int kmc_batch_results(int32_t batch, double* time, int64_t* eo, uint64_t* occ, int32_t* status,
+                      int64_t* hops_done, double* occ_time);
+int kmc_batch_destroy(int32_t batch);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* KMC_DN_AMD_H */

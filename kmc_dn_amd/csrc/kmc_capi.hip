@@ -1,0 +1,793 @@
+// Host side of libkmc_dn_amd.so: the C-ABI declared in include/kmc_dn_amd.h.
+//
+// Holds the device-resident registry (layouts, energy sets, batches) of one
+// engine (one device, one HIP stream), builds the per-layout tables the
+// kernels read, groups trajectories by layout for the batched kernel and
+// launches the kernels (optionally through a captured hipGraph).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "kmc_types.h"
+
+namespace kmc {
+
+template <bool REC>
+__global__ void kmc_plain_fp64_kernel(PlainParams);
+template <int C, int W, bool REC>
+__global__ void kmc_batch_fp32_kernel(BatchParams);
+__global__ void kmc_queue_reset_kernel(int32_t*);
+size_t plain_lds_bytes(int S);
+size_t batch_lds_bytes(int C, int W, int S);
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIPOK(expr)                                                                     \
+  do {                                                                                  \
+    hipError_t e_ = (expr);                                                             \
+    if (e_ != hipSuccess)                                                               \
+      return fail(KMC_ERR_DEVICE, "%s failed: %s", #expr, hipGetErrorString(e_));      \
+  } while (0)
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t n = 0;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  int alloc(size_t bytes) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = bytes;
+    if (bytes == 0) return KMC_OK;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+      p = nullptr;
+      return fail(KMC_ERR_NOMEM, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+    }
+    return KMC_OK;
+  }
+  template <class T>
+  T* as() const {
+    return static_cast<T*>(p);
+  }
+};
+
+struct Layout {
+  int N = 0, P = 0, S = 0;
+  DevBuf tab32, coul64, tc64, ird64, invd64;
+  LayoutDev dev{};
+};
+
+struct Batch {
+  int64_t ntraj = 0;
+  std::vector<TrajDev> host;        // in layout-grouped order
+  std::vector<int64_t> perm;        // grouped position -> caller index
+  int smax = 0;
+  DevBuf trajs, out, occ_time, queue, groups;
+  int32_t ngroups = 0;
+  int last_W = -1;
+  bool last_rec = false;
+  bool has_occ_time = false;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipGraphExec_t graph = nullptr;
+  int graph_key = -1;
+  ~Batch() {
+    if (graph) (void)hipGraphExecDestroy(graph);
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+  }
+};
+
+struct Engine {
+  std::mutex mu;
+  bool ready = false;
+  int device = -1;
+  int cu_count = 0;
+  int lds_bytes = 0;
+  char name[256] = {0};
+  hipStream_t stream = nullptr;
+  std::vector<Layout*> layouts;
+  std::vector<EsetDev> esets_host;
+  std::vector<int> eset_layout;
+  std::vector<DevBuf*> eset_bufs;
+  DevBuf layouts_dev, esets_dev;
+  std::vector<Batch*> batches;
+  bool layouts_dirty = true, esets_dirty = true;
+};
+
+Engine& eng() {
+  static Engine e;
+  return e;
+}
+
+int ensure_init() {
+  Engine& E = eng();
+  if (E.ready) return KMC_OK;
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return fail(KMC_ERR_DEVICE, "no HIP device: %s", hipGetErrorString(e));
+  E.device = dev;
+  hipDeviceProp_t prop;
+  HIPOK(hipGetDeviceProperties(&prop, dev));
+  E.cu_count = prop.multiProcessorCount;
+  E.lds_bytes = (int)prop.sharedMemPerBlock;
+  snprintf(E.name, sizeof E.name, "%s", prop.gcnArchName);
+  HIPOK(hipStreamCreateWithFlags(&E.stream, hipStreamNonBlocking));
+  E.ready = true;
+  return KMC_OK;
+}
+
+int sync_registry() {
+  Engine& E = eng();
+  if (E.layouts_dirty) {
+    std::vector<LayoutDev> v;
+    v.reserve(E.layouts.size());
+    for (Layout* L : E.layouts) v.push_back(L ? L->dev : LayoutDev{});
+    int st = E.layouts_dev.alloc(std::max<size_t>(1, v.size()) * sizeof(LayoutDev));
+    if (st) return st;
+    if (!v.empty()) HIPOK(hipMemcpy(E.layouts_dev.p, v.data(), v.size() * sizeof(LayoutDev), hipMemcpyHostToDevice));
+    E.layouts_dirty = false;
+  }
+  if (E.esets_dirty) {
+    int st = E.esets_dev.alloc(std::max<size_t>(1, E.esets_host.size()) * sizeof(EsetDev));
+    if (st) return st;
+    if (!E.esets_host.empty())
+      HIPOK(hipMemcpy(E.esets_dev.p, E.esets_host.data(), E.esets_host.size() * sizeof(EsetDev),
+                      hipMemcpyHostToDevice));
+    E.esets_dirty = false;
+  }
+  return KMC_OK;
+}
+
+template <class T>
+int upload(DevBuf& b, const std::vector<T>& v) {
+  int st = b.alloc(v.size() * sizeof(T));
+  if (st) return st;
+  if (!v.empty()) HIPOK(hipMemcpy(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  return KMC_OK;
+}
+
+int layout_create_locked(int64_t N, int64_t P, double nu, double kT, double I_0, double R,
+                         const double* d, const double* tc, double prune, int32_t* out) {
+  if (N < 0 || P < 0 || N > NMAX || P > PMAX || N + P <= 0)
+    return fail(KMC_ERR_LIMIT, "network N=%lld P=%lld outside 0<=N<=%d, 0<=P<=%d",
+                (long long)N, (long long)P, NMAX, PMAX);
+  if (N + P > SMAX_FP32)
+    return fail(KMC_ERR_LIMIT, "S=N+P=%lld exceeds %d sites", (long long)(N + P), SMAX_FP32);
+  if (!d || !tc || !out) return fail(KMC_ERR_ARGS, "null table pointer");
+  if (!(kT > 0) || !(nu > 0) || !std::isfinite(I_0) || !std::isfinite(R))
+    return fail(KMC_ERR_ARGS, "need kT > 0, nu > 0 and finite I_0, R");
+  const int S = (int)(N + P);
+  const int n = (int)N;
+  const double I0R = I_0 * R;
+  const double scale2 = 1.4426950408889634 / kT;
+  double tcmax = 0.0;
+  for (int k = 0; k < S * S; ++k) tcmax = std::max(tcmax, tc[k]);
+  const double cut = prune > 0 ? prune * tcmax : -1.0;
+  for (int i = 0; i < S; ++i)
+    for (int j = 0; j < i; ++j) {
+      const double a = tc[i * S + j], b = tc[j * S + i];
+      if (std::fabs(a - b) > 1e-12 * std::max(std::fabs(a), std::fabs(b)))
+        return fail(KMC_ERR_ARGS, "transitions_constant is not symmetric at (%d,%d)", i, j);
+      if (i < n && j < n && !(d[i * S + j] > 0))
+        return fail(KMC_ERR_ARGS, "acceptors %d and %d coincide (distance %g)", i, j, d[i * S + j]);
+    }
+  std::vector<float2> tab((size_t)S * S + 64 * 3);
+  for (auto& x : tab) x = make_float2(-INFINITY, 0.f);
+  std::vector<double> coul((size_t)n * n, 0.0), tc64(tc, tc + (size_t)S * S),
+      ird((size_t)S * S, 0.0), invd((size_t)n * n, 0.0);
+  const double lnu = std::log2(nu);
+  for (int u = 0; u < S; ++u)
+    for (int s = 0; s < S; ++s) {
+      const bool both_el = s >= n && u >= n;
+      const double t = tc[s * S + u];
+      float lt = -INFINITY;
+      if (s != u && !both_el && t > 0 && t > cut) lt = (float)(lnu + std::log2(t));
+      float c = 0.f;
+      if (s < n && u < n && s != u) c = (float)(scale2 * (I0R / d[s * S + u]));
+      tab[(size_t)u * S + s] = make_float2(lt, c);
+    }
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j)
+      if (i != j) {
+        coul[(size_t)j * n + i] = I0R / d[i * S + j];
+        ird[(size_t)i * S + j] = I_0 * R / d[i * S + j];
+        invd[(size_t)j * n + i] = 1.0 / d[i * S + j];
+      }
+  Layout* L = new Layout();
+  L->N = n;
+  L->P = (int)P;
+  L->S = S;
+  int st;
+  if ((st = upload(L->tab32, tab)) || (st = upload(L->coul64, coul)) || (st = upload(L->tc64, tc64)) ||
+      (st = upload(L->ird64, ird)) || (st = upload(L->invd64, invd))) {
+    delete L;
+    return st;
+  }
+  LayoutDev& D = L->dev;
+  D.N = n;
+  D.P = (int)P;
+  D.S = S;
+  D.prune = prune > 0;
+  D.nu = nu;
+  D.kT = kT;
+  D.I0 = I_0;
+  D.R = R;
+  D.I0R = I0R;
+  D.scale2 = (float)scale2;
+  D.tab32 = L->tab32.as<float2>();
+  D.coul64 = L->coul64.as<double>();
+  D.tc64 = L->tc64.as<double>();
+  D.ird64 = L->ird64.as<double>();
+  D.invd64 = L->invd64.as<double>();
+  Engine& E = eng();
+  E.layouts.push_back(L);
+  E.layouts_dirty = true;
+  *out = (int32_t)(E.layouts.size() - 1);
+  return KMC_OK;
+}
+
+int eset_create_locked(int32_t layout, int64_t count, const double* Ec, const double* V,
+                       int32_t* first) {
+  Engine& E = eng();
+  if (layout < 0 || layout >= (int)E.layouts.size() || !E.layouts[layout])
+    return fail(KMC_ERR_ARGS, "bad layout id %d", layout);
+  if (count <= 0 || !first) return fail(KMC_ERR_ARGS, "bad energy-set count");
+  const Layout* L = E.layouts[layout];
+  const size_t nE = (size_t)count * L->N, nV = (size_t)count * L->P;
+  if ((nE && !Ec) || (nV && !V)) return fail(KMC_ERR_ARGS, "null E_constant or V");
+  for (size_t i = 0; i < nE; ++i)
+    if (!std::isfinite(Ec[i])) return fail(KMC_ERR_ARGS, "E_constant[%zu] is not finite", i);
+  for (size_t i = 0; i < nV; ++i)
+    if (!std::isfinite(V[i])) return fail(KMC_ERR_ARGS, "V[%zu] is not finite", i);
+  DevBuf* b = new DevBuf();
+  int st = b->alloc((nE + nV) * sizeof(double) + 16);
+  if (st) {
+    delete b;
+    return st;
+  }
+  if (nE) HIPOK(hipMemcpy(b->p, Ec, nE * sizeof(double), hipMemcpyHostToDevice));
+  if (nV) HIPOK(hipMemcpy(b->as<double>() + nE, V, nV * sizeof(double), hipMemcpyHostToDevice));
+  *first = (int32_t)E.esets_host.size();
+  for (int64_t i = 0; i < count; ++i) {
+    EsetDev e;
+    e.E = b->as<double>() + (size_t)i * L->N;
+    e.V = b->as<double>() + nE + (size_t)i * L->P;
+    E.esets_host.push_back(e);
+    E.eset_layout.push_back(layout);
+  }
+  E.eset_bufs.push_back(b);
+  E.esets_dirty = true;
+  return KMC_OK;
+}
+
+Batch* get_batch(int32_t id) {
+  Engine& E = eng();
+  if (id < 0 || id >= (int)E.batches.size()) return nullptr;
+  return E.batches[id];
+}
+
+int batch_create_locked(const kmc_traj* t, int64_t ntraj, int32_t* out) {
+  Engine& E = eng();
+  if (!t || ntraj <= 0 || !out) return fail(KMC_ERR_ARGS, "empty batch");
+  if (ntraj > (1 << 30)) return fail(KMC_ERR_LIMIT, "too many trajectories");
+  Batch* B = new Batch();
+  B->ntraj = ntraj;
+  B->perm.resize(ntraj);
+  std::iota(B->perm.begin(), B->perm.end(), 0);
+  for (int64_t i = 0; i < ntraj; ++i) {
+    const kmc_traj& x = t[i];
+    if (x.layout < 0 || x.layout >= (int)E.layouts.size() || !E.layouts[x.layout] || x.eset < 0 ||
+        x.eset >= (int)E.esets_host.size() || E.eset_layout[x.eset] != x.layout) {
+      delete B;
+      return fail(KMC_ERR_ARGS, "trajectory %lld: bad layout/energy-set pair (%d, %d)",
+                  (long long)i, x.layout, x.eset);
+    }
+    if (x.hops < 0 || x.prehops < 0 || x.hops + x.prehops > (int64_t)INT32_MAX) {
+      delete B;
+      return fail(KMC_ERR_ARGS, "trajectory %lld: hops out of range", (long long)i);
+    }
+    const int N = E.layouts[x.layout]->N;
+    for (int w = 0; w < OCCW; ++w) {
+      const int lo = w * 64;
+      uint64_t valid = N <= lo ? 0 : (N - lo >= 64 ? ~0ull : ((1ull << (N - lo)) - 1));
+      if (x.occ[w] & ~valid) {
+        delete B;
+        return fail(KMC_ERR_ARGS, "trajectory %lld: occupation bit beyond N=%d", (long long)i, N);
+      }
+    }
+  }
+  // group by layout, longest budgets first inside each layout
+  std::stable_sort(B->perm.begin(), B->perm.end(), [&](int64_t a, int64_t b) {
+    if (t[a].layout != t[b].layout) return t[a].layout < t[b].layout;
+    return t[a].prehops + t[a].hops > t[b].prehops + t[b].hops;
+  });
+  B->host.resize(ntraj);
+  for (int64_t i = 0; i < ntraj; ++i) {
+    const kmc_traj& x = t[B->perm[i]];
+    TrajDev& d = B->host[i];
+    d.layout = x.layout;
+    d.eset = x.eset;
+    d.seed = x.seed;
+    d.hop0 = x.hop0;
+    d.prehops = x.prehops;
+    d.hops = x.hops;
+    for (int w = 0; w < OCCW; ++w) d.occ[w] = x.occ[w];
+    B->smax = std::max(B->smax, E.layouts[x.layout]->S);
+  }
+  int st;
+  if ((st = upload(B->trajs, B->host)) || (st = B->out.alloc(sizeof(TrajOut) * ntraj)) ||
+      (st = B->queue.alloc(64))) {
+    delete B;
+    return st;
+  }
+  if (hipEventCreate(&B->ev0) != hipSuccess || hipEventCreate(&B->ev1) != hipSuccess) {
+    delete B;
+    return fail(KMC_ERR_DEVICE, "hipEventCreate failed");
+  }
+  E.batches.push_back(B);
+  *out = (int32_t)(E.batches.size() - 1);
+  return KMC_OK;
+}
+
+// Groups of <= W consecutive same-layout trajectories, ordered by decreasing budget.
+int build_groups(Batch* B, int W) {
+  std::vector<GroupDev> g;
+  int64_t i = 0;
+  while (i < B->ntraj) {
+    int64_t j = i;
+    while (j < B->ntraj && j - i < W && B->host[j].layout == B->host[i].layout) ++j;
+    GroupDev gd;
+    gd.layout = B->host[i].layout;
+    gd.first = (int32_t)i;
+    gd.count = (int32_t)(j - i);
+    gd.pad = 0;
+    g.push_back(gd);
+    i = j;
+  }
+  std::stable_sort(g.begin(), g.end(), [&](const GroupDev& a, const GroupDev& b) {
+    const TrajDev& x = B->host[a.first];
+    const TrajDev& y = B->host[b.first];
+    return x.prehops + x.hops > y.prehops + y.hops;
+  });
+  B->ngroups = (int32_t)g.size();
+  return upload(B->groups, g);
+}
+
+using BatchKernel = void (*)(BatchParams);
+
+BatchKernel pick_batch_kernel(int C, int W, bool rec) {
+#define KMC_PICK(CC, WW)                                                                     \
+  if (C == CC && W == WW)                                                                    \
+    return rec ? (BatchKernel)kmc_batch_fp32_kernel<CC, WW, true>                           \
+               : (BatchKernel)kmc_batch_fp32_kernel<CC, WW, false>;
+  KMC_PICK(1, 1) KMC_PICK(1, 2) KMC_PICK(1, 4) KMC_PICK(1, 8)
+  KMC_PICK(2, 1) KMC_PICK(2, 2) KMC_PICK(2, 4) KMC_PICK(2, 8)
+  KMC_PICK(3, 1) KMC_PICK(3, 2) KMC_PICK(3, 4) KMC_PICK(3, 8)
+#undef KMC_PICK
+  return nullptr;
+}
+
+struct Fp32Launch {
+  BatchKernel k = nullptr;
+  int grid = 0, block = 0;
+  size_t lds = 0;
+  BatchParams p{};
+};
+
+// Everything that allocates or queries happens here, outside any graph capture.
+int prepare_fp32(Batch* B, const kmc_run_opts& o, Fp32Launch& L) {
+  Engine& E = eng();
+  const int C = (B->smax + 63) / 64;
+  int W = o.waves_per_group;
+  if (W <= 0) W = C == 1 ? 4 : (C == 2 ? 2 : 1);
+  if (W != 1 && W != 2 && W != 4 && W != 8) return fail(KMC_ERR_ARGS, "waves_per_group must be 1, 2, 4 or 8");
+  const bool rec = o.record != 0;
+  BatchKernel k = pick_batch_kernel(C, W, rec);
+  if (!k) return fail(KMC_ERR_LIMIT, "no kernel for %d site chunks", C);
+  const size_t lds = batch_lds_bytes(C, W, B->smax);
+  if (lds > (size_t)160 * 1024)
+    return fail(KMC_ERR_LIMIT, "LDS %zu bytes for S=%d, W=%d exceeds 160 KiB", lds, B->smax, W);
+  HIPOK(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  if (W != B->last_W) {
+    int st = build_groups(B, W);
+    if (st) return st;
+    B->last_W = W;
+  }
+  int per_cu = 0;
+  HIPOK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k, 64 * W, lds));
+  if (per_cu <= 0) return fail(KMC_ERR_LIMIT, "kernel does not fit on a CU (LDS %zu)", lds);
+  int grid = per_cu * E.cu_count;
+  if (o.grid_limit > 0) grid = std::min(grid, o.grid_limit);
+  grid = std::max(1, std::min(grid, (int)B->ngroups));
+  if (rec && !B->has_occ_time) {
+    int st = B->occ_time.alloc(sizeof(double) * NMAX * B->ntraj);
+    if (st) return st;
+    B->has_occ_time = true;
+  }
+  BatchParams& p = L.p;
+  p.trajs = B->trajs.as<TrajDev>();
+  p.layouts = E.layouts_dev.as<LayoutDev>();
+  p.esets = E.esets_dev.as<EsetDev>();
+  p.groups = B->groups.as<GroupDev>();
+  p.ngroups = B->ngroups;
+  p.recompute_interval = o.recompute_interval > 0 ? o.recompute_interval : 1024;
+  p.queue = B->queue.as<int32_t>();
+  p.out = B->out.as<TrajOut>();
+  p.occ_time = rec ? B->occ_time.as<double>() : nullptr;
+  p.post_hop_record = o.post_hop_record;
+  L.k = k;
+  L.grid = grid;
+  L.block = 64 * W;
+  L.lds = lds;
+  return KMC_OK;
+}
+
+int launch_fp32(Fp32Launch& L, hipStream_t s) {
+  int32_t* q = L.p.queue;
+  void* qa[] = {&q};
+  HIPOK(hipLaunchKernel((const void*)kmc_queue_reset_kernel, dim3(1), dim3(1), qa, 0, s));
+  void* args[] = {&L.p};
+  HIPOK(hipLaunchKernel((const void*)L.k, dim3(L.grid), dim3(L.block), args, L.lds, s));
+  return KMC_OK;
+}
+
+int enqueue_fp64(Batch* B, const kmc_run_opts& o, hipStream_t s, const double* uniforms,
+                 int32_t* events, double* totals, double* traffic, double* se_out) {
+  Engine& E = eng();
+  if (B->smax > SMAX_FP64)
+    return fail(KMC_ERR_LIMIT, "FP64 dense kernel supports S <= %d (got %d)", SMAX_FP64, B->smax);
+  const bool rec = o.record != 0 || traffic != nullptr;
+  if (rec && !B->has_occ_time) {
+    int st = B->occ_time.alloc(sizeof(double) * NMAX * B->ntraj);
+    if (st) return st;
+    B->has_occ_time = true;
+  }
+  const size_t lds = plain_lds_bytes(B->smax);
+  auto k = rec ? kmc_plain_fp64_kernel<true> : kmc_plain_fp64_kernel<false>;
+  HIPOK(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  PlainParams p{};
+  p.trajs = B->trajs.as<TrajDev>();
+  p.layouts = E.layouts_dev.as<LayoutDev>();
+  p.esets = E.esets_dev.as<EsetDev>();
+  p.out = B->out.as<TrajOut>();
+  p.uniforms = uniforms;
+  p.events = events;
+  p.totals = totals;
+  p.traffic = traffic;
+  p.occ_time = rec ? B->occ_time.as<double>() : nullptr;
+  p.se_out = se_out;
+  void* args[] = {&p};
+  HIPOK(hipLaunchKernel((const void*)k, dim3((unsigned)B->ntraj), dim3(64), args, lds, s));
+  return KMC_OK;
+}
+
+int batch_run_locked(int32_t id, const kmc_run_opts* opts) {
+  Engine& E = eng();
+  Batch* B = get_batch(id);
+  if (!B) return fail(KMC_ERR_ARGS, "bad batch id %d", id);
+  kmc_run_opts o{};
+  if (opts) o = *opts;
+  int st = sync_registry();
+  if (st) return st;
+  hipStream_t s = E.stream;
+  if (o.precision == KMC_FP64_DENSE) {
+    HIPOK(hipEventRecord(B->ev0, s));
+    st = enqueue_fp64(B, o, s, nullptr, nullptr, nullptr, nullptr, nullptr);
+    if (st) return st;
+    HIPOK(hipEventRecord(B->ev1, s));
+    B->last_rec = o.record != 0;
+    return KMC_OK;
+  }
+  if (o.precision != KMC_FP32_BATCHED) return fail(KMC_ERR_ARGS, "unknown precision %d", o.precision);
+  Fp32Launch L;
+  if ((st = prepare_fp32(B, o, L))) return st;
+  if (o.use_graph) {
+    const int key = (o.waves_per_group & 0xFF) | ((o.record & 1) << 8) | ((o.post_hop_record & 1) << 9) |
+                    ((o.recompute_interval & 0xFFFFF) << 10);
+    if (!B->graph || B->graph_key != key || o.grid_limit) {
+      if (B->graph) {
+        (void)hipGraphExecDestroy(B->graph);
+        B->graph = nullptr;
+      }
+      // Capture on a private stream (capture mode "relaxed" leaves other threads' work alone).
+      hipStream_t cs;
+      HIPOK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+      hipGraph_t g;
+      HIPOK(hipStreamBeginCapture(cs, hipStreamCaptureModeRelaxed));
+      st = launch_fp32(L, cs);
+      hipError_t ce = hipStreamEndCapture(cs, &g);
+      (void)hipStreamDestroy(cs);
+      if (st) return st;
+      if (ce != hipSuccess) return fail(KMC_ERR_DEVICE, "graph capture: %s", hipGetErrorString(ce));
+      hipError_t ie = hipGraphInstantiate(&B->graph, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (ie != hipSuccess) return fail(KMC_ERR_DEVICE, "graph instantiate: %s", hipGetErrorString(ie));
+      B->graph_key = key;
+    }
+    HIPOK(hipEventRecord(B->ev0, s));
+    HIPOK(hipGraphLaunch(B->graph, s));
+    HIPOK(hipEventRecord(B->ev1, s));
+  } else {
+    HIPOK(hipEventRecord(B->ev0, s));
+    st = launch_fp32(L, s);
+    if (st) return st;
+    HIPOK(hipEventRecord(B->ev1, s));
+  }
+  B->last_rec = o.record != 0;
+  return KMC_OK;
+}
+
+}  // namespace
+}  // namespace kmc
+
+using namespace kmc;
+
+extern "C" {
+
+const char* kmc_last_error(void) { return g_err.c_str(); }
+
+int kmc_version(void) { return 1; }
+
+int kmc_init(int device) {
+  Engine& E = eng();
+  std::lock_guard<std::mutex> lk(E.mu);
+  if (E.ready) {
+    if (device >= 0 && device != E.device)
+      return fail(KMC_ERR_ARGS, "engine already bound to device %d", E.device);
+    return KMC_OK;
+  }
+  if (device >= 0) HIPOK(hipSetDevice(device));
+  return ensure_init();
+}
+
+int kmc_device_info(char* name, int name_len, int* cu_count, int* lds_bytes) {
+  Engine& E = eng();
+  std::lock_guard<std::mutex> lk(E.mu);
+  int st = ensure_init();
+  if (st) return st;
+  if (name && name_len > 0) snprintf(name, (size_t)name_len, "%s", E.name);
+  if (cu_count) *cu_count = E.cu_count;
+  if (lds_bytes) *lds_bytes = E.lds_bytes;
+  return KMC_OK;
+}
+
+int kmc_reset(void) {
+  Engine& E = eng();
+  std::lock_guard<std::mutex> lk(E.mu);
+  if (E.ready) (void)hipStreamSynchronize(E.stream);
+  for (Batch* b : E.batches) delete b;
+  E.batches.clear();
+  for (Layout* l : E.layouts) delete l;
+  E.layouts.clear();
+  for (DevBuf* b : E.eset_bufs) delete b;
+  E.eset_bufs.clear();
+  E.esets_host.clear();
+  E.eset_layout.clear();
+  E.layouts_dirty = E.esets_dirty = true;
+  return KMC_OK;
+}
+
+int kmc_layout_create(int64_t N, int64_t P, double nu, double kT, double I_0, double R,
+                      const double* distances, const double* transitions_constant,
+                      double prune_threshold, int32_t* layout_out) {
+  Engine& E = eng();
+  std::lock_guard<std::mutex> lk(E.mu);
+  int st = ensure_init();
+  if (st) return st;
+  return layout_create_locked(N, P, nu, kT, I_0, R, distances, transitions_constant,
+                              prune_threshold, layout_out);
+}
+
+int kmc_eset_create(int32_t layout, int64_t count, const double* E_constant, const double* V,
+                    int32_t* first_out) {
+  Engine& E = eng();
+  std::lock_guard<std::mutex> lk(E.mu);
+  int st = ensure_init();
+  if (st) return st;
+  return eset_create_locked(layout, count, E_constant, V, first_out);
+}
+
+int kmc_batch_create(const kmc_traj* trajs, int64_t ntraj, int32_t* batch_out) {
+  Engine& E = eng();
+  std::lock_guard<std::mutex> lk(E.mu);
+  int st = ensure_init();
+  if (st) return st;
+  return batch_create_locked(trajs, ntraj, batch_out);
+}
+
+int kmc_batch_run(int32_t batch, const kmc_run_opts* opts) {
+  Engine& E = eng();
+  std::lock_guard<std::mutex> lk(E.mu);
+  int st = ensure_init();
+  if (st) return st;
+  return batch_run_locked(batch, opts);
+}
+
+int kmc_batch_sync(int32_t batch, double* elapsed_ms) {
+  Engine& E = eng();
+  Batch* B;
+  {
+    std::lock_guard<std::mutex> lk(E.mu);
+    B = get_batch(batch);
+    if (!B) return fail(KMC_ERR_ARGS, "bad batch id %d", batch);
+  }
+  HIPOK(hipEventSynchronize(B->ev1));
+  if (elapsed_ms) {
+    float ms = 0.f;
+    HIPOK(hipEventElapsedTime(&ms, B->ev0, B->ev1));
+    *elapsed_ms = ms;
+  }
+  return KMC_OK;
+}
+
+int kmc_batch_results(int32_t batch, double* time, int64_t* eo, uint64_t* occ, int32_t* status,
+                      int64_t* hops_done, double* occ_time) {
+  Engine& E = eng();
+  std::lock_guard<std::mutex> lk(E.mu);
+  Batch* B = get_batch(batch);
+  if (!B) return fail(KMC_ERR_ARGS, "bad batch id %d", batch);
+  HIPOK(hipStreamSynchronize(E.stream));
+  std::vector<TrajOut> out(B->ntraj);
+  HIPOK(hipMemcpy(out.data(), B->out.p, sizeof(TrajOut) * B->ntraj, hipMemcpyDeviceToHost));
+  std::vector<double> ot;
+  if (occ_time) {
+    if (!B->has_occ_time || !B->last_rec) return fail(KMC_ERR_ARGS, "batch was not run with record=1");
+    ot.resize((size_t)NMAX * B->ntraj);
+    HIPOK(hipMemcpy(ot.data(), B->occ_time.p, sizeof(double) * ot.size(), hipMemcpyDeviceToHost));
+  }
+  for (int64_t i = 0; i < B->ntraj; ++i) {
+    const int64_t j = B->perm[i];   // caller's index
+    const TrajOut& r = out[i];
+    const int P = eng().layouts[B->host[i].layout]->P;
+    if (time) time[j] = r.time;
+    if (status) status[j] = r.status;
+    if (hops_done) hops_done[j] = r.hops_done;
+    if (occ)
+      for (int w = 0; w < OCCW; ++w) occ[j * OCCW + w] = r.occ[w];
+    if (eo)
+      for (int e = 0; e < PMAX; ++e) eo[j * PMAX + e] = e < P ? r.eo[e] : 0;
+    if (occ_time) memcpy(occ_time + (size_t)j * NMAX, ot.data() + (size_t)i * NMAX, sizeof(double) * NMAX);
+  }
+  return KMC_OK;
+}
+
+int kmc_batch_destroy(int32_t batch) {
+  Engine& E = eng();
+  std::lock_guard<std::mutex> lk(E.mu);
+  Batch* B = get_batch(batch);
+  if (!B) return fail(KMC_ERR_ARGS, "bad batch id %d", batch);
+  (void)hipStreamSynchronize(E.stream);
+  delete B;
+  E.batches[batch] = nullptr;
+  return KMC_OK;
+}
+
+int kmc_simulate_discrete(int64_t N, int64_t P, double nu, double kT, double I_0, double R,
+                          const double* distances, const double* E_constant,
+                          double* site_energies, const double* transitions_constant,
+                          uint8_t* occupation, int64_t* electrode_occupation, int64_t hops,
+                          int record, double* traffic, double* occupations_in_time,
+                          const double* uniforms, uint64_t seed, uint64_t hop0,
+                          int32_t* events_out, double* totals_out, double* time_out) {
+  Engine& E = eng();
+  std::lock_guard<std::mutex> lk(E.mu);
+  int st = ensure_init();
+  if (st) return st;
+  if (!site_energies || !occupation || (P > 0 && !electrode_occupation) || !E_constant)
+    return fail(KMC_ERR_ARGS, "null argument");
+  if (hops < 0 || hops > INT32_MAX) return fail(KMC_ERR_ARGS, "hops out of range");
+  if (N + P > SMAX_FP64)
+    return fail(KMC_ERR_LIMIT, "FP64 dense kernel supports S <= %d (got %lld)", SMAX_FP64, (long long)(N + P));
+  int32_t lid, eid, bid;
+  if ((st = layout_create_locked(N, P, nu, kT, I_0, R, distances, transitions_constant, 0.0, &lid)))
+    return st;
+  // The layout/energy set/batch are private to this call and released below.
+  auto cleanup = [&](int code) {
+    (void)hipStreamSynchronize(E.stream);
+    if (bid >= 0 && bid < (int)E.batches.size()) {
+      delete E.batches[bid];
+      E.batches.pop_back();
+    }
+    if (!E.eset_bufs.empty() && eid >= 0) {
+      delete E.eset_bufs.back();
+      E.eset_bufs.pop_back();
+      E.esets_host.pop_back();
+      E.eset_layout.pop_back();
+      E.esets_dirty = true;
+    }
+    delete E.layouts.back();
+    E.layouts.pop_back();
+    E.layouts_dirty = true;
+    return code;
+  };
+  eid = -1;
+  bid = -1;
+  if ((st = eset_create_locked(lid, 1, E_constant, site_energies + N, &eid))) return cleanup(st);
+  kmc_traj tr{};
+  tr.layout = lid;
+  tr.eset = eid;
+  tr.seed = seed;
+  tr.hop0 = hop0;
+  tr.prehops = 0;
+  tr.hops = hops;
+  for (int64_t i = 0; i < N; ++i)
+    if (occupation[i]) tr.occ[i >> 6] |= 1ull << (i & 63);
+  int32_t b;
+  if ((st = batch_create_locked(&tr, 1, &b))) return cleanup(st);
+  bid = b;
+  if ((st = sync_registry())) return cleanup(st);
+  Batch* B = E.batches[bid];
+  DevBuf du, dev_ev, dtot, dtr, dse;
+  const int S = (int)(N + P);
+  if (uniforms) {
+    if ((st = du.alloc(sizeof(double) * 2 * hops))) return cleanup(st);
+    if (hops) HIPOK(hipMemcpy(du.p, uniforms, sizeof(double) * 2 * hops, hipMemcpyHostToDevice));
+  }
+  if (events_out && (st = dev_ev.alloc(sizeof(int32_t) * hops))) return cleanup(st);
+  if (totals_out && (st = dtot.alloc(sizeof(double) * hops))) return cleanup(st);
+  if (record && traffic) {
+    if ((st = dtr.alloc(sizeof(double) * S * S))) return cleanup(st);
+    HIPOK(hipMemset(dtr.p, 0, sizeof(double) * S * S));
+  }
+  if ((st = dse.alloc(sizeof(double) * NMAX))) return cleanup(st);
+  kmc_run_opts o{};
+  o.precision = KMC_FP64_DENSE;
+  o.record = record ? 1 : 0;
+  o.post_hop_record = 1;
+  if ((st = enqueue_fp64(B, o, E.stream, uniforms ? du.as<double>() : nullptr,
+                         events_out ? dev_ev.as<int32_t>() : nullptr,
+                         totals_out ? dtot.as<double>() : nullptr,
+                         (record && traffic) ? dtr.as<double>() : nullptr, dse.as<double>())))
+    return cleanup(st);
+  hipError_t se = hipStreamSynchronize(E.stream);
+  if (se != hipSuccess) return cleanup(fail(KMC_ERR_DEVICE, "kernel: %s", hipGetErrorString(se)));
+  TrajOut r;
+  HIPOK(hipMemcpy(&r, B->out.p, sizeof r, hipMemcpyDeviceToHost));
+  for (int64_t i = 0; i < N; ++i) occupation[i] = (r.occ[i >> 6] >> (i & 63)) & 1ull;
+  for (int64_t e = 0; e < P; ++e) electrode_occupation[e] += r.eo[e];
+  if (time_out) *time_out = r.time;
+  std::vector<double> sev(NMAX);
+  HIPOK(hipMemcpy(sev.data(), dse.p, sizeof(double) * NMAX, hipMemcpyDeviceToHost));
+  if (hops > 0)
+    for (int64_t i = 0; i < N; ++i) site_energies[i] = sev[i];
+  if (events_out && hops) HIPOK(hipMemcpy(events_out, dev_ev.p, sizeof(int32_t) * hops, hipMemcpyDeviceToHost));
+  if (totals_out && hops) HIPOK(hipMemcpy(totals_out, dtot.p, sizeof(double) * hops, hipMemcpyDeviceToHost));
+  if (record) {
+    if (traffic) HIPOK(hipMemcpy(traffic, dtr.p, sizeof(double) * S * S, hipMemcpyDeviceToHost));
+    if (occupations_in_time) {
+      std::vector<double> ot(NMAX);
+      HIPOK(hipMemcpy(ot.data(), B->occ_time.p, sizeof(double) * NMAX, hipMemcpyDeviceToHost));
+      for (int64_t i = 0; i < N; ++i) occupations_in_time[i] = ot[i];
+    }
+  }
+  const int status = r.status;
+  cleanup(KMC_OK);
+  if (status != KMC_OK)
+    return fail(status, "total rate became zero after %lld hops (reference: ZeroDivisionError)",
+                (long long)r.hops_done);
+  return KMC_OK;
+}
+
+}  // extern "C"

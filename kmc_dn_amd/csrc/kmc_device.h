@@ -1,0 +1,137 @@
+// Device-side building blocks shared by the KMC kernels (gfx950, wave64).
+//
+//  * Philox4x32-10 counter-based RNG and the engine's stream contract:
+//    key = trajectory seed, counter = (hop lo, hop hi, 0, 0); words 0,1 give
+//    the clock uniform, words 2,3 the event uniform (53-bit, [0,1)).
+//  * Wave-wide inclusive scans in FP64 and FP32 built on DPP row shifts and
+//    row broadcasts (no LDS round trip).
+//  * Ballot / mbcnt / readlane helpers.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "kmc_types.h"
+
+namespace kmc {
+
+// ----------------------------------------------------------------------------
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11)
+// ----------------------------------------------------------------------------
+struct u32x4 {
+  uint32_t x, y, z, w;
+};
+
+__device__ __forceinline__ u32x4 philox4x32_10(u32x4 c, uint32_t k0, uint32_t k1) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    if (r) {
+      k0 += 0x9E3779B9u;
+      k1 += 0xBB67AE85u;
+    }
+    const uint32_t lo0 = 0xD2511F53u * c.x;
+    const uint32_t hi0 = __umulhi(0xD2511F53u, c.x);
+    const uint32_t lo1 = 0xCD9E8D57u * c.z;
+    const uint32_t hi1 = __umulhi(0xCD9E8D57u, c.z);
+    c = u32x4{hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0};
+  }
+  return c;
+}
+
+__device__ __forceinline__ double u53(uint32_t a, uint32_t b) {
+  return ((double)(a >> 5) * 67108864.0 + (double)(b >> 6)) * (1.0 / 9007199254740992.0);
+}
+
+// Uniforms of hop `hop` of the stream keyed by `seed`.
+__device__ __forceinline__ void stream_uniforms(uint64_t seed, uint64_t hop, double& u_clock,
+                                                double& u_event) {
+  const u32x4 o = philox4x32_10(u32x4{(uint32_t)hop, (uint32_t)(hop >> 32), 0u, 0u},
+                                (uint32_t)seed, (uint32_t)(seed >> 32));
+  u_clock = u53(o.x, o.y);
+  u_event = u53(o.z, o.w);
+}
+
+// ----------------------------------------------------------------------------
+// Wave helpers
+// ----------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
+
+__device__ __forceinline__ int mbcnt(uint64_t m) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
+}
+
+__device__ __forceinline__ float readlane_f32(float v, int lane) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
+__device__ __forceinline__ int readlane_i32(int v, int lane) {
+  return __builtin_amdgcn_readlane(v, lane);
+}
+
+__device__ __forceinline__ double readlane_f64(double v, int lane) {
+  const uint64_t b = __double_as_longlong(v);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, lane);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), lane);
+  return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
+// Orders this wave's LDS accesses (writes by some lanes, reads by others).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// DPP controls (GFX9 encoding).
+enum : int {
+  DPP_ROW_SHR1 = 0x111,
+  DPP_ROW_SHR2 = 0x112,
+  DPP_ROW_SHR4 = 0x114,
+  DPP_ROW_SHR8 = 0x118,
+  DPP_ROW_BCAST15 = 0x142,
+  DPP_ROW_BCAST31 = 0x143,
+};
+
+template <int CTRL, int ROW_MASK, bool BOUND>
+__device__ __forceinline__ float dpp_f32(float x) {
+  return __int_as_float(
+      __builtin_amdgcn_update_dpp(0, __float_as_int(x), CTRL, ROW_MASK, 0xF, BOUND));
+}
+
+template <int CTRL, int ROW_MASK, bool BOUND>
+__device__ __forceinline__ double dpp_f64(double x) {
+  const uint64_t b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)b, CTRL, ROW_MASK, 0xF, BOUND);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)(b >> 32), CTRL, ROW_MASK, 0xF, BOUND);
+  return __longlong_as_double((long long)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo));
+}
+
+// Inclusive scan over the 64 lanes of a wave (Hillis-Steele inside each row of
+// 16 lanes, then row broadcasts).  Every lane must be active.
+__device__ __forceinline__ double wave_scan_f64(double x) {
+  x += dpp_f64<DPP_ROW_SHR1, 0xF, true>(x);
+  x += dpp_f64<DPP_ROW_SHR2, 0xF, true>(x);
+  x += dpp_f64<DPP_ROW_SHR4, 0xF, true>(x);
+  x += dpp_f64<DPP_ROW_SHR8, 0xF, true>(x);
+  x += dpp_f64<DPP_ROW_BCAST15, 0xA, false>(x);
+  x += dpp_f64<DPP_ROW_BCAST31, 0xC, false>(x);
+  return x;
+}
+
+__device__ __forceinline__ float wave_scan_f32(float x) {
+  x += dpp_f32<DPP_ROW_SHR1, 0xF, true>(x);
+  x += dpp_f32<DPP_ROW_SHR2, 0xF, true>(x);
+  x += dpp_f32<DPP_ROW_SHR4, 0xF, true>(x);
+  x += dpp_f32<DPP_ROW_SHR8, 0xF, true>(x);
+  x += dpp_f32<DPP_ROW_BCAST15, 0xA, false>(x);
+  x += dpp_f32<DPP_ROW_BCAST31, 0xC, false>(x);
+  return x;
+}
+
+__device__ __forceinline__ float fast_exp2(float x) { return __builtin_amdgcn_exp2f(x); }
+__device__ __forceinline__ float fast_log2(float x) { return __builtin_amdgcn_logf(x); }
+
+__device__ __forceinline__ bool occ_bit(const uint64_t* occ, int i) {
+  return (occ[i >> 6] >> (i & 63)) & 1ull;
+}
+
+}  // namespace kmc

@@ -1,0 +1,106 @@
+// Plain-old-data records shared by the host library and the kernels.
+//
+// A *layout* is one network geometry (acceptor + electrode positions and the
+// constant tables derived from them); an *energy set* is one set of site
+// constants E_constant[N] and electrode voltages V[P] on a layout; a
+// *trajectory* is one independent KMC run (layout, energy set, RNG key, hop
+// budget, initial occupation).
+#pragma once
+#include <stdint.h>
+
+#include "../../include/kmc_dn_amd.h"
+
+namespace kmc {
+
+constexpr int NMAX = KMC_NMAX;   // acceptors per network
+constexpr int PMAX = KMC_PMAX;   // electrodes per network
+constexpr int OCCW = KMC_OCCW;   // 64-bit words of an occupation bitmask
+constexpr int SMAX_FP32 = 192;   // sites of the FP32 batched kernel (3 lanes per site chunk)
+constexpr int SMAX_FP64 = 80;    // sites of the FP64 dense kernel (LDS-resident lists)
+
+struct LayoutDev {
+  int32_t N, P, S, prune;
+  double nu, kT, I0, R;
+  double I0R;        // I_0 * R
+  float scale2;      // log2(e) / kT: energies in kT units, base-2 exponent
+  float pad0;
+  // FP32 batched kernel.  tab32[u * S + s] = {lt, c} for the ordered pair
+  // (s, u), symmetric in (s, u):
+  //   lt = log2(nu * transitions_constant[s][u])  (-inf when impossible:
+  //        diagonal, electrode-electrode, pruned or underflowed),
+  //   c  = scale2 * I_0 * R / d[s][u] for acceptor-acceptor pairs, else 0.
+  // 64 * 3 trailing entries {-inf, 0} let lanes past S read in bounds.
+  const float2* tab32;
+  const double* coul64;   // [N*N] I_0*R/d for acceptor pairs (symmetric, 0 diagonal)
+  // FP64 dense kernel (reference arithmetic).
+  const double* tc64;     // [S*S] transitions_constant
+  const double* ird64;    // [S*S] (I_0*R)/d[i][j] (acceptor pairs; 0 elsewhere)
+  const double* invd64;   // [N*N] 1.0/d[i][j] (0 on the diagonal)
+};
+
+struct EsetDev {
+  const double* E;   // [N] E_constant
+  const double* V;   // [P] electrode voltages (site_energies[N:])
+};
+
+struct TrajDev {
+  int32_t layout;
+  int32_t eset;
+  uint64_t seed;      // Philox key
+  uint64_t hop0;      // Philox counter of the first hop
+  int64_t prehops;    // hops run before tallies start (not counted)
+  int64_t hops;       // counted hops
+  uint64_t occ[OCCW]; // initial occupation bitmask
+};
+
+struct TrajOut {
+  double time;
+  int64_t hops_done;
+  int32_t status;
+  int32_t pad;
+  uint64_t occ[OCCW];
+  int64_t eo[PMAX];
+};
+
+struct GroupDev {
+  int32_t layout;
+  int32_t first;   // first trajectory (trajectories of a group are contiguous)
+  int32_t count;   // <= waves per workgroup
+  int32_t pad;
+};
+
+struct BatchParams {
+  const TrajDev* trajs;
+  const LayoutDev* layouts;
+  const EsetDev* esets;
+  const GroupDev* groups;
+  int32_t ngroups;
+  int32_t recompute_interval;   // hops between FP64 energy recomputes (>0)
+  int32_t* queue;               // work-queue head, zero before launch
+  TrajOut* out;
+  double* occ_time;             // [ntraj * NMAX] or null
+  int32_t post_hop_record;      // 1: numba convention (post-hop state)
+  int32_t pad;
+};
+
+struct PlainParams {
+  const TrajDev* trajs;
+  const LayoutDev* layouts;
+  const EsetDev* esets;
+  TrajOut* out;
+  const double* uniforms;   // replayed stream (2 per hop) for trajectory 0, or null
+  int32_t* events;          // [prehops+hops] event index from*S+to, or null
+  double* totals;           // [prehops+hops] problist[-1] per hop, or null
+  double* traffic;          // [ntraj * S*S] gross traffic, or null
+  double* occ_time;         // [ntraj * NMAX] post-hop occupation time, or null
+  double* se_out;           // [ntraj * NMAX] final site_energies[0:N], or null
+};
+
+// Status codes written per trajectory (also returned by the C-ABI).
+enum : int32_t {
+  ST_OK = KMC_OK,
+  ST_ZERO_RATE = KMC_ERR_ZERO_RATE,
+  ST_BAD_INPUT = KMC_ERR_ARGS,
+};
+
+}  // namespace kmc

@@ -1,0 +1,200 @@
+"""Python view of the device engine: layouts, energy sets, batches.
+
+This mirrors what the reference does with ``parrallelSimulation``
+(goSimulation/parrallelSimulationBind.py:9-79) -- many independent networks in
+one native call -- but keeps everything resident on the GPU: a :class:`Layout`
+(geometry tables) is uploaded once, energy sets (E_constant, electrode
+voltages) are uploaded per operating point, and a :class:`Batch` of
+trajectories is launched, timed and read back as numpy arrays.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib as L
+
+TRAJ_DTYPE = np.dtype([("layout", np.int32), ("eset", np.int32), ("seed", np.uint64),
+                       ("hop0", np.uint64), ("prehops", np.int64), ("hops", np.int64),
+                       ("occ", np.uint64, (L.OCCW,))])
+assert TRAJ_DTYPE.itemsize == C.sizeof(L.kmc_traj)
+
+
+def init(device: int = -1) -> None:
+    L.check(L.load().kmc_init(device))
+
+
+def device_info() -> dict:
+    lib = L.load()
+    name = C.create_string_buffer(256)
+    cu, lds = C.c_int(), C.c_int()
+    L.check(lib.kmc_device_info(name, 256, C.byref(cu), C.byref(lds)))
+    return {"name": name.value.decode(), "cu_count": cu.value, "lds_bytes": lds.value}
+
+
+def reset() -> None:
+    L.check(L.load().kmc_reset())
+
+
+def occupation_to_bits(occ) -> np.ndarray:
+    """Boolean occupation (..., N) -> (..., OCCW) uint64 bitmask words."""
+    occ = np.asarray(occ, dtype=bool)
+    N = occ.shape[-1]
+    if N > L.NMAX:
+        raise ValueError(f"N={N} exceeds {L.NMAX}")
+    pad = np.zeros(occ.shape[:-1] + (L.OCCW * 64,), dtype=bool)
+    pad[..., :N] = occ
+    b = pad.reshape(occ.shape[:-1] + (L.OCCW, 64)).astype(np.uint64)
+    return (b << np.arange(64, dtype=np.uint64)).sum(axis=-1, dtype=np.uint64)
+
+
+def bits_to_occupation(bits, N: int) -> np.ndarray:
+    bits = np.asarray(bits, dtype=np.uint64)
+    x = (bits[..., :, None] >> np.arange(64, dtype=np.uint64)) & np.uint64(1)
+    return x.reshape(bits.shape[:-1] + (L.OCCW * 64,))[..., :N].astype(bool)
+
+
+class Layout:
+    """One network geometry on the device (kmc_layout_create)."""
+
+    def __init__(self, N: int, P: int, nu: float, kT: float, I_0: float, R: float,
+                 distances, transitions_constant, prune_threshold: float = 0.0):
+        self.N, self.P, self.S = int(N), int(P), int(N) + int(P)
+        self.nu, self.kT, self.I_0, self.R = float(nu), float(kT), float(I_0), float(R)
+        d = np.ascontiguousarray(distances, dtype=np.float64)
+        tc = np.ascontiguousarray(transitions_constant, dtype=np.float64)
+        if d.shape != (self.S, self.S) or tc.shape != (self.S, self.S):
+            raise ValueError(f"tables must be {self.S}x{self.S}")
+        lid = C.c_int32()
+        L.check(L.load().kmc_layout_create(self.N, self.P, self.nu, self.kT, self.I_0, self.R,
+                                           L.ptr(d, L._dp), L.ptr(tc, L._dp),
+                                           float(prune_threshold), C.byref(lid)))
+        self.id = lid.value
+
+    def energy_sets(self, E_constant, V) -> np.ndarray:
+        """Upload energy sets (rows of E_constant [k,N] and V [k,P]); returns their ids."""
+        E = np.ascontiguousarray(np.atleast_2d(E_constant), dtype=np.float64)
+        Vv = np.ascontiguousarray(np.asarray(V, dtype=np.float64).reshape(E.shape[0], self.P))
+        if E.shape[1] != self.N:
+            raise ValueError("E_constant rows must have N entries")
+        first = C.c_int32()
+        L.check(L.load().kmc_eset_create(self.id, E.shape[0], L.ptr(E, L._dp), L.ptr(Vv, L._dp),
+                                         C.byref(first)))
+        return np.arange(first.value, first.value + E.shape[0], dtype=np.int32)
+
+
+@dataclass
+class BatchResult:
+    time: np.ndarray          # [K]
+    eo: np.ndarray            # [K, PMAX] int64 electrode tallies
+    occ_bits: np.ndarray      # [K, OCCW]
+    status: np.ndarray        # [K]
+    hops_done: np.ndarray     # [K]
+    occ_time: np.ndarray | None = None   # [K, NMAX]
+
+    def currents(self, P: int) -> np.ndarray:
+        return self.eo[:, :P] / self.time[:, None]
+
+
+def make_trajs(n: int, layout: int | np.ndarray, eset: int | np.ndarray, seeds, hops,
+               occupation, prehops=0, hop0=0) -> np.ndarray:
+    t = np.zeros(n, dtype=TRAJ_DTYPE)
+    t["layout"] = layout
+    t["eset"] = eset
+    t["seed"] = seeds
+    t["hop0"] = hop0
+    t["prehops"] = prehops
+    t["hops"] = hops
+    t["occ"] = occupation_to_bits(occupation) if np.asarray(occupation).dtype == bool \
+        else occupation
+    return t
+
+
+class Batch:
+    """A device-resident set of trajectories (kmc_batch_create/run/sync/results)."""
+
+    def __init__(self, trajs: np.ndarray):
+        trajs = np.ascontiguousarray(trajs, dtype=TRAJ_DTYPE)
+        self.n = trajs.shape[0]
+        bid = C.c_int32()
+        L.check(L.load().kmc_batch_create(trajs.ctypes.data_as(C.POINTER(L.kmc_traj)), self.n,
+                                          C.byref(bid)))
+        self.id = bid.value
+        self.recorded = False
+
+    def run(self, precision: int = L.FP32_BATCHED, waves_per_group: int = 0,
+            recompute_interval: int = 0, record: bool = False, post_hop_record: bool = False,
+            grid_limit: int = 0, use_graph: bool = False) -> None:
+        o = L.kmc_run_opts(int(precision), int(waves_per_group), int(recompute_interval),
+                           int(record), int(post_hop_record), int(grid_limit), int(use_graph), 0)
+        L.check(L.load().kmc_batch_run(self.id, C.byref(o)))
+        self.recorded = bool(record)
+
+    def sync(self) -> float:
+        ms = C.c_double()
+        L.check(L.load().kmc_batch_sync(self.id, C.byref(ms)))
+        return ms.value
+
+    def results(self) -> BatchResult:
+        n = self.n
+        t = np.zeros(n)
+        eo = np.zeros((n, L.PMAX), dtype=np.int64)
+        occ = np.zeros((n, L.OCCW), dtype=np.uint64)
+        st = np.zeros(n, dtype=np.int32)
+        hd = np.zeros(n, dtype=np.int64)
+        ot = np.zeros((n, L.NMAX)) if self.recorded else None
+        L.check(L.load().kmc_batch_results(self.id, L.ptr(t, L._dp), L.ptr(eo, L._i64p),
+                                           L.ptr(occ, L._u64p), L.ptr(st, L._i32p),
+                                           L.ptr(hd, L._i64p), L.ptr(ot, L._dp)))
+        return BatchResult(t, eo, occ, st, hd, ot)
+
+    def close(self) -> None:
+        if self.id >= 0:
+            L.check(L.load().kmc_batch_destroy(self.id))
+            self.id = -1
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def simulate_discrete(N, P, nu, kT, I_0, R, distances, E_constant, site_energies,
+                      transitions_constant, occupation, electrode_occupation, hops, record=False,
+                      uniforms=None, seed=0, hop0=0, log=False):
+    """One network through kmc_simulate_discrete (FP64 dense kernel, one launch).
+
+    ``site_energies``, ``occupation`` and ``electrode_occupation`` are updated in
+    place like the reference's numba function updates its arguments.  Returns
+    (time, traffic, occupations_in_time, events, totals).
+    """
+    S = int(N) + int(P)
+    d = np.ascontiguousarray(distances, dtype=np.float64)
+    tc = np.ascontiguousarray(transitions_constant, dtype=np.float64)
+    E = np.ascontiguousarray(E_constant, dtype=np.float64)
+    se = site_energies
+    if se.dtype != np.float64 or not se.flags.c_contiguous:
+        raise TypeError("site_energies must be a contiguous float64 array")
+    occ = np.ascontiguousarray(occupation, dtype=np.uint8)
+    eo = np.ascontiguousarray(electrode_occupation, dtype=np.int64)
+    traffic = np.zeros((S, S)) if record else None
+    oit = np.zeros(int(N)) if record else None
+    u = None if uniforms is None else np.ascontiguousarray(uniforms, dtype=np.float64).ravel()
+    if u is not None and u.size < 2 * hops:
+        raise ValueError("need 2 uniforms per hop")
+    ev = np.zeros(hops, dtype=np.int32) if log else None
+    tot = np.zeros(hops) if log else None
+    t = C.c_double()
+    rc = L.load().kmc_simulate_discrete(
+        int(N), int(P), float(nu), float(kT), float(I_0), float(R), L.ptr(d, L._dp),
+        L.ptr(E, L._dp), L.ptr(se, L._dp), L.ptr(tc, L._dp), L.ptr(occ, L._u8p),
+        L.ptr(eo, L._i64p), int(hops), int(bool(record)), L.ptr(traffic, L._dp),
+        L.ptr(oit, L._dp), L.ptr(u, L._dp), int(seed), int(hop0), L.ptr(ev, L._i32p),
+        L.ptr(tot, L._dp), C.byref(t))
+    occupation[...] = occ.astype(occupation.dtype)
+    electrode_occupation[...] = eo
+    L.check(rc)
+    return t.value, traffic, oit, ev, tot

@@ -1,0 +1,113 @@
+"""FP32-rate / FP64-accumulate batched kernel: determinism, invariants, statistics."""
+import numpy as np
+import pytest
+
+from kmc_dn_amd import _lib, engine
+from tests.netgen import fixture_network, random_network
+
+pytestmark = pytest.mark.gpu
+
+
+def _setup(net, K, hops, seed0=0, prehops=0):
+    N, P = net["N"], net["P"]
+    lay = engine.Layout(N, P, net["nu"], net["kT"], net["I_0"], net["R"], net["distances"],
+                        net["transitions_constant"])
+    es = lay.energy_sets(net["E_constant"], net["site_energies"][N:])
+    seeds = np.arange(seed0, seed0 + K, dtype=np.uint64)
+    return lay, engine.Batch(engine.make_trajs(K, lay.id, es[0], seeds, hops, net["occupation"],
+                                               prehops=prehops))
+
+
+def test_results_independent_of_grouping_and_graph(gpu_engine):
+    net = random_network(0, 30, 8, M=3, I_0=50.0, a=0.1, vmax=1.0)
+    _, b = _setup(net, 100, 2000)
+    outs = []
+    for W, graph in [(1, False), (2, False), (4, False), (8, False), (4, True), (4, True)]:
+        print(f"[gpu] fp32 W={W} graph={graph}", flush=True)
+        b.run(precision=_lib.FP32_BATCHED, waves_per_group=W, use_graph=graph)
+        b.sync()
+        outs.append(b.results())
+    for r in outs[1:]:
+        assert np.array_equal(r.eo, outs[0].eo)
+        assert np.array_equal(r.time, outs[0].time)
+        assert np.array_equal(r.occ_bits, outs[0].occ_bits)
+
+
+def test_invariants(gpu_engine):
+    net = random_network(2, 40, 6, I_0=50.0, a=0.1, vmax=1.5)
+    N, P = 40, 6
+    _, b = _setup(net, 256, 5000)
+    b.run(precision=_lib.FP32_BATCHED)
+    b.sync()
+    r = b.results()
+    assert (r.status == 0).all() and (r.hops_done == 5000).all()
+    holes0 = net["occupation"].sum()
+    holes = engine.bits_to_occupation(r.occ_bits, N).sum(axis=1)
+    # every hop moves one hole: what the electrodes absorbed left the acceptors
+    assert np.array_equal(r.eo[:, :P].sum(axis=1), holes0 - holes)
+    assert (r.eo[:, P:] == 0).all()
+    assert (r.time > 0).all()
+
+
+def _ensemble(net, precision, K, hops, **kw):
+    _, b = _setup(net, K, hops, prehops=kw.pop("prehops", 0))
+    b.run(precision=precision, **kw)
+    ms = b.sync()
+    r = b.results()
+    assert (r.status == 0).all()
+    return r.currents(net["P"]), ms
+
+
+def _compare(ca, cb, label, nsig=4.0, floor=0.0):
+    ma, mb = ca.mean(0), cb.mean(0)
+    se = np.sqrt(ca.var(0, ddof=1) / len(ca) + cb.var(0, ddof=1) / len(cb))
+    z = np.abs(ma - mb) / np.maximum(se, 1e-300)
+    print(f"[gpu] {label}: max |z| {z.max():.2f}; rel diff "
+          f"{np.max(np.abs(ma - mb) / np.maximum(np.abs(mb), 1e-300)):.2e}", flush=True)
+    assert np.all((z < nsig) | (np.abs(ma - mb) < floor)), (label, ma, mb, se)
+
+
+def test_fp32_matches_fp64_statistics_c1_like(gpu_engine):
+    net = random_network(0, 30, 8, M=3, I_0=50.0, a=0.1, vmax=1.0)
+    c64, ms64 = _ensemble(net, _lib.FP64_DENSE, 512, 20000, prehops=2000)
+    c32, ms32 = _ensemble(net, _lib.FP32_BATCHED, 2048, 20000, prehops=2000)
+    print(f"[gpu] fp64 {512 * 22000 / ms64 / 1e3:.3e} hops/s, fp32 {2048 * 22000 / ms32 / 1e3:.3e}"
+          " hops/s", flush=True)
+    _compare(c32, c64, "fp32 vs fp64 (C1-like)")
+
+
+def test_recompute_interval_does_not_bias(gpu_engine):
+    net = random_network(4, 50, 8, I_0=50.0, a=0.1, vmax=1.0)
+    c1, _ = _ensemble(net, _lib.FP32_BATCHED, 1024, 10000, recompute_interval=1)
+    c2, _ = _ensemble(net, _lib.FP32_BATCHED, 1024, 10000, recompute_interval=1 << 20)
+    _compare(c1, c2, "recompute every hop vs never")
+
+
+@pytest.mark.parametrize("N,P", [(60, 10), (100, 12), (120, 12)])
+def test_multi_chunk_networks(gpu_engine, N, P):
+    net = random_network(N, N, P, I_0=50.0, a=0.1, vmax=1.0)
+    c32, _ = _ensemble(net, _lib.FP32_BATCHED, 512, 5000, prehops=1000)
+    if N + P <= 80:
+        c64, _ = _ensemble(net, _lib.FP64_DENSE, 256, 5000, prehops=1000)
+        _compare(c32, c64, f"fp32 vs fp64 N={N} P={P}")
+    else:
+        assert np.isfinite(c32).all()
+
+
+def test_fixture_currents_against_reference(fixtures, gpu_engine):
+    """Ensemble means vs the reference's own mean/std over 5 runs of 1e6 hops."""
+    zs = []
+    for lset, sname in [("rnd", "rnd_min_max"), ("xor", "XOR_wide")]:
+        for k in range(0, 100, 10):
+            net = fixture_network(fixtures, lset, k)
+            c, _ = _ensemble(net, _lib.FP32_BATCHED, 64, 100000)
+            m, se = c.mean(0), c.std(0, ddof=1) / np.sqrt(len(c))
+            rm = fixtures[f"{sname}_mean_currents"][k]
+            rs = fixtures[f"{sname}_stddev_currents"][k]
+            z = np.abs(m - rm) / np.sqrt(se ** 2 + rs ** 2 / 4)
+            zs.append(z[np.abs(rm) > 1e-5])
+    z = np.concatenate(zs)
+    print(f"[gpu] fixtures: median |z| {np.median(z):.2f}, 95% {np.percentile(z, 95):.2f}, "
+          f"frac>3 {np.mean(z > 3):.3f}", flush=True)
+    # the reference's own numba-vs-Go comparison: median 0.74, p95 2.6, 3% above 3
+    assert np.median(z) < 1.2 and np.mean(z > 3) < 0.08
