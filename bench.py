@@ -1,0 +1,206 @@
+#!/usr/bin/env python3
+"""Benchmark of the KMC hop loop on one MI355X.
+
+Headline: hops/s of the FP32-rate / FP64-accumulate batched kernel on the
+4096-trajectory ensemble of the N=30 network (30 acceptors, 3 donors, 8
+perimeter electrodes, a = 0.1, I_0 = 50 kT, V ~ U(-1, 1), Laplace potential,
+Philox keys 0..4095, 1e5 hops per trajectory).  One step = one launch over the
+whole ensemble (4.096e8 hops) with every input already resident in HBM.
+
+Beside it, on the same layout and in the same run:
+  * the plain FP64 path (reference arithmetic, one trajectory per launch);
+  * the LDS roofline of the batched kernel (algorithmic LDS bytes per hop of
+    the reference's dense formulation / kernel time from HIP events);
+  * the CPU baseline: the oracle's restatement of the numba loop, ensemble-
+    parallel over this host's cores on a bounded sample (plus the Go-loop
+    restatement).
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+METRIC = "hops/sec, 4096-traj N=30 ensemble vs plain; p50/p99 latency of served IV jobs"
+# MI355X: 256 CUs at up to 2.4 GHz; LDS 160 KiB per CU delivering 128 bytes per clock
+# to the SIMDs (ds_read_b128 path) -> 256 * 128 * 2.4e9 B/s.
+LDS_PEAK_GBS = 256 * 128 * 2.4
+# HBM bytes per launch of the batched kernel measured with rocprofv3 --pmc FETCH_SIZE
+# and WRITE_SIZE on this workload, scaled by the copy-kernel calibration factor
+# (profiles/kmc_batch_fp32_c1_hbm_traffic.md); None until measured.
+HBM_TRAFFIC_BYTES = None
+
+
+def lds_bytes_per_hop(N: int, P: int) -> int:
+    """Algorithmic LDS bytes of one hop in the reference's dense formulation.
+
+    4*S^2 (FP32 log-prefactor per ordered pair) + 4*N^2 (FP32 Coulomb kernel per
+    acceptor pair) + 4*S (site energies) + 8*N (two Coulomb columns of the
+    rank-1 update); S = N + P.
+    """
+    S = N + P
+    return 4 * S * S + 4 * N * N + 4 * S + 8 * N
+
+
+def host_cores() -> int:
+    n = len(os.sched_getaffinity(0))
+    cap = os.environ.get("OMP_NUM_THREADS")
+    if cap and cap.isdigit() and int(cap) > 0:
+        n = min(n, int(cap))
+    return max(1, n)
+
+
+def cpu_baselines(ens, seconds: float) -> tuple[dict, dict]:
+    """Oracle restatements timed on this host (numba loop FP64; Go loop FP32)."""
+    import oracle
+    net = ens.net
+    cores = host_cores()
+    args = net.oracle_args()
+    # calibrate the per-core rate on a short single-thread run
+    t0 = time.perf_counter()
+    oracle.numba_ensemble(*args, net.occupation, np.arange(1), 1000, nthreads=1)
+    per_core = 1000 / (time.perf_counter() - t0)
+    ntraj = 4 * cores
+    hops = max(1000, int(per_core * seconds / 4))
+    seeds = ens.seeds[:ntraj]
+    t0 = time.perf_counter()
+    _, _, _, st = oracle.numba_ensemble(*args, net.occupation, seeds, hops, nthreads=cores)
+    dt = time.perf_counter() - t0
+    numba = {"value": ntraj * hops / dt, "unit": "hops/s", "cores": cores, "kind": "port",
+             "sample": f"oracle restatement of the numba loop (FP64 dense, "
+                       f"kmc_dopant_networks.py:33-135), {ntraj} trajectories x {hops} hops of "
+                       f"the same layout, OpenMP one trajectory per thread, {dt:.1f} s",
+             "status_ok": bool((st == 0).all())}
+    t0 = time.perf_counter()
+    occ0 = np.zeros(net.N, np.uint8)
+    _, _, _, st = oracle.go_ensemble(*args, occ0, seeds, hops, nthreads=cores)
+    dt = time.perf_counter() - t0
+    go = {"value": ntraj * hops / dt, "unit": "hops/s", "cores": cores, "kind": "port",
+          "sample": f"oracle restatement of the Go loop (FP32, incremental energies, no cache, "
+                    f"simulation.go:194-325), {ntraj} x {hops} hops, {dt:.1f} s",
+          "status_ok": bool((st == 0).all())}
+    return numba, go
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--ntraj", type=int, default=4096)
+    ap.add_argument("--hops", type=int, default=100_000)
+    ap.add_argument("--waves-per-group", type=int, default=0)
+    ap.add_argument("--recompute-interval", type=int, default=1024)
+    ap.add_argument("--plain-launches", type=int, default=4)
+    ap.add_argument("--plain-hops", type=int, default=2000)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} (WORLD_SIZE={world}) is not part of "
+              "this build: trajectories are independent, so N GPUs would each run their own share "
+              "of the ensemble with no exchange.", file=sys.stderr)
+        return 2
+
+    from kmc_dn_amd import _lib, engine, workloads
+
+    engine.init(0)
+    info = engine.device_info()
+    ens = workloads.ensemble(seed=0, K=args.ntraj, hops=args.hops)
+    net = ens.net
+    lay = engine.Layout(net.N, net.P, net.nu, net.kT, net.I_0, net.R, net.distances,
+                        net.transitions_constant)
+    es = lay.energy_sets(net.E_constant, net.site_energies[net.N:])
+    batch = engine.Batch(engine.make_trajs(args.ntraj, lay.id, es[0], ens.seeds, args.hops,
+                                           net.occupation))
+    run = dict(precision=_lib.FP32_BATCHED, waves_per_group=args.waves_per_group,
+               recompute_interval=args.recompute_interval)
+    for _ in range(args.warmup):
+        batch.run(**run)
+        batch.sync()
+    kernel_ms = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        batch.run(**run)
+        kernel_ms.append(batch.sync())
+    wall = time.perf_counter() - t0
+    res = batch.results()
+    assert (res.status == 0).all() and (res.hops_done == args.hops).all()
+    hops_per_step = args.ntraj * args.hops
+    value = hops_per_step * args.steps / wall
+    avg_kernel_s = float(np.mean(kernel_ms)) * 1e-3
+    achieved = lds_bytes_per_hop(net.N, net.P) * hops_per_step / avg_kernel_s / 1e9
+    cur = res.currents(net.P)
+
+    # plain FP64 path: one trajectory per launch
+    plain = engine.Batch(engine.make_trajs(1, lay.id, es[0], ens.seeds[:1], args.plain_hops,
+                                           net.occupation))
+    plain.run(precision=_lib.FP64_DENSE)
+    plain.sync()
+    pms = []
+    tp = time.perf_counter()
+    for _ in range(args.plain_launches):
+        plain.run(precision=_lib.FP64_DENSE)
+        pms.append(plain.sync())
+    pwall = time.perf_counter() - tp
+    plain_value = args.plain_hops * args.plain_launches / pwall
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "hops/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: seeded layout (RandomState(0)), Philox trajectory streams",
+        "config": {
+            "workload": "4096-trajectory ensemble of one 2D network: N=30 acceptors, M=3 donors, "
+                        "P=8 perimeter electrodes, a=0.1 (ab=a*R), I_0=50 kT, V~U(-1,1) kT/e, "
+                        "P1 Laplace potential, 1e5 hops per trajectory",
+            "n_traj": args.ntraj, "hops_per_traj": args.hops, "N": net.N, "P": net.P,
+            "kernel": "fp32 rates / fp64 scan, clock and tallies; one wave per trajectory",
+            "recompute_interval": args.recompute_interval, "device": info["name"],
+        },
+        "roofline": {
+            "bound": "lds",
+            "achieved": achieved,
+            "peak": LDS_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / LDS_PEAK_GBS,
+            "traffic": HBM_TRAFFIC_BYTES,
+            "bytes_per_hop": lds_bytes_per_hop(net.N, net.P),
+            "kernel_ms": float(np.mean(kernel_ms)),
+        },
+        "plain_fp64": {"value": plain_value, "unit": "hops/s",
+                       "kernel_ms": float(np.mean(pms)),
+                       "sample": f"{args.plain_launches} launches x {args.plain_hops} hops, "
+                                 "one trajectory per launch, FP64 dense reference arithmetic"},
+        "speedup_vs_plain_fp64": value / plain_value,
+        "mean_current": [float(x) for x in cur.mean(0)],
+    }
+    if not args.no_cpu_baseline:
+        numba, go = cpu_baselines(ens, args.cpu_seconds)
+        out["cpu_baseline"] = numba
+        out["cpu_baseline_go"] = go
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -129,6 +129,15 @@ int kmc_batch_results(int32_t batch, double* time, int64_t* eo, uint64_t* occ, i
                       int64_t* hops_done, double* occ_time);
 int kmc_batch_destroy(int32_t batch);
 
+/* ---- measurement probes (no counterpart in the reference; used by bench.py) ---- */
+
+/* Plain device-to-device copy of `bytes`, timed over `repeats` launches; the
+ * kernel whose FETCH_SIZE/WRITE_SIZE calibrates rocprof's HBM byte counters. */
+int kmc_probe_copy(int64_t bytes, int repeats, double* ms_per_copy);
+/* LDS read bandwidth (bytes/s) of `blocks` x 256 threads streaming
+ * width_bytes (4, 8 or 16) reads for `iters` x 8 rounds. */
+int kmc_probe_lds(int width_bytes, int blocks, int iters, double* bytes_per_s);
+
 #ifdef __cplusplus
 }
 #endif

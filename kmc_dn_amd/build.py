@@ -31,6 +31,7 @@ SOURCES = {
     "kmc_plain.hip": ["-ffp-contract=off"],
     "kmc_batch.hip": [],
     "kmc_capi.hip": [],
+    "kmc_probe.hip": [],
 }
 
 

@@ -1,0 +1,108 @@
+// Measurement probes: a plain copy kernel (calibrates rocprof's FETCH_SIZE /
+// WRITE_SIZE against a known byte count) and an LDS read-bandwidth kernel
+// (measures the LDS peak the hop-loop kernel is compared with).
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "kmc_types.h"
+
+namespace kmc {
+
+__global__ void __launch_bounds__(256) kmc_probe_copy_kernel(const float4* __restrict__ src,
+                                                            float4* __restrict__ dst, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
+// Each thread streams WIDTH-byte LDS reads over a 64 KiB buffer `iters` times.
+template <int WIDTH>
+__global__ void __launch_bounds__(256) kmc_probe_lds_kernel(int iters, uint32_t* out) {
+  __shared__ __align__(16) uint32_t buf[16384];
+  for (int i = threadIdx.x; i < 16384; i += 256) buf[i] = i * 2654435761u;
+  __syncthreads();
+  constexpr int WORDS = WIDTH / 4;
+  uint32_t acc = 0;
+  int base = threadIdx.x * WORDS;
+  for (int it = 0; it < iters; ++it) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const int idx = (base + k * 256 * WORDS) & (16384 - WORDS);
+      if constexpr (WORDS == 4) {
+        const uint4 v = *reinterpret_cast<const uint4*>(&buf[idx]);
+        acc ^= v.x ^ v.y ^ v.z ^ v.w;
+      } else if constexpr (WORDS == 2) {
+        const uint2 v = *reinterpret_cast<const uint2*>(&buf[idx]);
+        acc ^= v.x ^ v.y;
+      } else {
+        acc ^= buf[idx];
+      }
+    }
+    base += 8 * 256 * WORDS + WORDS;
+  }
+  if (acc == 0x12345678u) out[blockIdx.x] = acc;   // keeps the loads live
+}
+
+}  // namespace kmc
+
+using namespace kmc;
+
+extern "C" {
+
+int kmc_probe_copy(int64_t bytes, int repeats, double* ms_per_copy) {
+  if (bytes <= 0 || repeats <= 0) return KMC_ERR_ARGS;
+  const size_t n = (size_t)bytes / sizeof(float4);
+  float4 *a = nullptr, *b = nullptr;
+  if (hipMalloc(&a, n * sizeof(float4)) != hipSuccess) return KMC_ERR_NOMEM;
+  if (hipMalloc(&b, n * sizeof(float4)) != hipSuccess) {
+    (void)hipFree(a);
+    return KMC_ERR_NOMEM;
+  }
+  (void)hipMemset(a, 1, n * sizeof(float4));
+  hipEvent_t e0, e1;
+  (void)hipEventCreate(&e0);
+  (void)hipEventCreate(&e1);
+  hipLaunchKernelGGL(kmc_probe_copy_kernel, dim3(4096), dim3(256), 0, 0, a, b, n);
+  (void)hipEventRecord(e0, 0);
+  for (int r = 0; r < repeats; ++r)
+    hipLaunchKernelGGL(kmc_probe_copy_kernel, dim3(4096), dim3(256), 0, 0, a, b, n);
+  (void)hipEventRecord(e1, 0);
+  hipError_t st = hipEventSynchronize(e1);
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  if (ms_per_copy) *ms_per_copy = ms / repeats;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(a);
+  (void)hipFree(b);
+  return st == hipSuccess ? KMC_OK : KMC_ERR_DEVICE;
+}
+
+int kmc_probe_lds(int width_bytes, int blocks, int iters, double* bytes_per_s) {
+  if (blocks <= 0 || iters <= 0) return KMC_ERR_ARGS;
+  uint32_t* out = nullptr;
+  if (hipMalloc(&out, sizeof(uint32_t) * blocks) != hipSuccess) return KMC_ERR_NOMEM;
+  void (*k)(int, uint32_t*) = width_bytes == 16 ? kmc_probe_lds_kernel<16>
+                              : width_bytes == 8 ? kmc_probe_lds_kernel<8>
+                                                 : kmc_probe_lds_kernel<4>;
+  const int w = width_bytes == 16 ? 16 : (width_bytes == 8 ? 8 : 4);
+  hipEvent_t e0, e1;
+  (void)hipEventCreate(&e0);
+  (void)hipEventCreate(&e1);
+  hipLaunchKernelGGL(k, dim3(blocks), dim3(256), 0, 0, 1, out);
+  (void)hipEventRecord(e0, 0);
+  hipLaunchKernelGGL(k, dim3(blocks), dim3(256), 0, 0, iters, out);
+  (void)hipEventRecord(e1, 0);
+  hipError_t st = hipEventSynchronize(e1);
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  if (bytes_per_s) *bytes_per_s = (double)blocks * 256 * 8 * (double)iters * w / (ms * 1e-3);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(out);
+  return st == hipSuccess ? KMC_OK : KMC_ERR_DEVICE;
+}
+
+}  // extern "C"

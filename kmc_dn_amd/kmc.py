@@ -1,0 +1,308 @@
+"""``kmc_dn`` -- the reference's model object, running its hop loop on the GPU.
+
+Mirrors ``class kmc_dn`` of kmc_dopant_networks.py:165-1081: same constructor,
+keyword arguments, attributes and methods, with the two hop-loop back ends
+replaced:
+
+* ``python_simulation`` (alias ``simulate_discrete``) runs the FP64 dense
+  kernel (the numba loop's arithmetic, kmc_dopant_networks.py:33-135) and
+  updates ``occupation``/``electrode_occupation`` in place, as numba does;
+* ``go_simulation`` runs the FP32 batched kernel with the Go library's
+  semantics (simulationWrapper.go:127-169: start from an empty network,
+  FP32 rates, pre-hop occupation time, net traffic).
+
+The Laplace solve (FEniCS in the reference) is kmc_dn_amd.laplace; random
+placement uses numpy's global generator exactly as the reference does (or a
+``rng=np.random.RandomState(...)`` keyword).  The RNG of the hop loop is the
+engine's Philox stream, keyed by a seed drawn from that same generator.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import _lib
+from . import engine
+from . import laplace
+from . import tables as T
+from .kmcfile import read_kmc, write_kmc
+
+
+class kmc_dn:
+    def __init__(self, N, M, xdim, ydim, zdim, mu=0, I_0=100, a=0.25, **kwargs):
+        """See kmc_dopant_networks.py:166-405 for the meaning of every argument."""
+        self._rng = kwargs.get("rng", None)
+        if "copy_from" in kwargs:
+            self.nu = kwargs["copy_from"].nu
+            self.kT = kwargs["copy_from"].kT
+            self.I_0 = kwargs["copy_from"].I_0
+        else:
+            self.nu = 1
+            self.kT = 1
+            self.I_0 = I_0 * self.kT
+        self.time = 0
+        self.mu = mu
+        self.N, self.M = N, M
+        self.xdim, self.ydim, self.zdim = xdim, ydim, zdim
+        self.dim, self.R = T.characteristic_length(N, xdim, ydim, zdim)
+        self.ab = a * self.R
+        if "electrodes" in kwargs:
+            self.electrodes = np.array(kwargs["electrodes"], dtype=np.float64, copy=True)
+            self.P = self.electrodes.shape[0]
+        else:
+            self.electrodes = np.zeros((0, 4))
+            self.P = 0
+        if "acceptors" in kwargs:
+            self.acceptors = np.array(kwargs["acceptors"], dtype=np.float64, copy=True)
+        if "donors" in kwargs:
+            self.donors = np.array(kwargs["donors"], dtype=np.float64, copy=True)
+        self.static_electrodes = (np.array(kwargs["static_electrodes"], dtype=np.float64)
+                                  if "static_electrodes" in kwargs else np.zeros((0, 4)))
+        if "res" in kwargs:
+            self.res = kwargs["res"]
+        elif self.dim == 1:
+            self.res = self.xdim / 100
+        elif self.dim == 2:
+            self.res = min([self.xdim, self.ydim]) / 100
+        else:
+            self.res = min([self.xdim, self.ydim, self.zdim]) / 100
+        self.calc_E_constant = self.calc_E_constant_V_comp
+        if kwargs.get("calc_E_constant") == "calc_E_constant_V":
+            self.calc_E_constant = self.calc_E_constant_V
+        self._device_layout = None
+        self.initialize(dopant_placement=not hasattr(self, "acceptors"),
+                        charge_placement=not hasattr(self, "donors"))
+
+    # ------------------------------------------------------------------ setup
+    @property
+    def rng(self):
+        return self._rng if self._rng is not None else np.random.mtrand._rand
+
+    def initialize(self, dopant_placement=True, charge_placement=True, distances=True, V=True,
+                   E_constant=True):
+        """kmc_dopant_networks.py:408-452."""
+        S = self.N + self.P
+        self.transitions = np.zeros((S, S))
+        self.transitions_constant = np.zeros((S, S))
+        self.distances = np.zeros((S, S))
+        self.vectors = np.zeros((S, S, 3))
+        self.site_energies = np.zeros((S,))
+        self.problist = np.zeros(S ** 2)
+        self.occupation = np.zeros(self.N, dtype=bool)
+        self.electrode_occupation = np.zeros(self.P, dtype=int)
+        if dopant_placement:
+            self.place_dopants_random()
+        if charge_placement:
+            self.place_charges_random()
+        if distances:
+            self.calc_distances()
+            self.calc_transitions_constant()
+        if V:
+            self.init_V()
+        if E_constant:
+            self.calc_E_constant()
+        self._device_layout = None
+
+    def reset(self):
+        """kmc_dopant_networks.py:454-471 (occupation is kept)."""
+        self.time = 0
+        self.old_current = 0
+        self.counter = 0
+        self.electrode_occupation = np.zeros(self.P, dtype=int)
+
+    def place_dopants_random(self):
+        self.acceptors, self.donors = T.place_dopants_random(self.rng, self.N, self.M, self.xdim,
+                                                             self.ydim, self.zdim)
+
+    def place_charges_random(self):
+        self.occupation = T.place_charges_random(self.rng, self.N, self.M)
+
+    def calc_distances(self):
+        self.distances = T.calc_distances(self.acceptors, self.electrodes)
+        pos = T.site_positions(self.acceptors, self.electrodes)
+        diff = pos[None, :, :] - pos[:, None, :]
+        with np.errstate(invalid="ignore", divide="ignore"):
+            self.vectors = np.where(self.distances[..., None] > 0,
+                                    diff / self.distances[..., None], 0.0)
+
+    def calc_transitions_constant(self):
+        self.transitions_constant = T.calc_transitions_constant(self.distances, self.ab, self.nu)
+
+    def update_electrodes(self, electrodes):
+        self.electrodes = electrodes
+        self.P = self.electrodes.shape[0]
+        self.initialize(dopant_placement=False, charge_placement=False)
+
+    def init_V(self):
+        """Sets up the potential solver (FEniCS in the reference, :706-796)."""
+        self._laplace = None
+        if self.dim == 2:
+            self._laplace = laplace.Laplace2D(self.xdim, self.ydim, self.res)
+        elif self.dim == 1:
+            self._laplace = laplace.Interval1D(self.xdim, self.res)
+        self._solve_V()
+
+    def _solve_V(self):
+        if self.dim == 2:
+            self.V_nodes = self._laplace.solve_electrodes(self.electrodes, self.static_electrodes,
+                                                          self.mu)
+        elif self.dim == 1:
+            self.V_nodes = self._laplace.solve_electrodes(self.electrodes, self.static_electrodes,
+                                                          self.mu)
+        else:
+            self.V_nodes = None
+
+    def V(self, x, y=None):
+        """Potential at a point, like the FEniCS Function self.V(x, y)."""
+        if self.dim == 1:
+            return float(self._laplace.evaluate(self.V_nodes, x))
+        return float(self._laplace.mesh.evaluate(self.V_nodes, x, y))
+
+    def update_V(self):
+        """kmc_dopant_networks.py:798-821."""
+        self._solve_V()
+        self.calc_E_constant()
+
+    def _eV(self):
+        if self.dim == 2:
+            return self._laplace.mesh.evaluate(self.V_nodes, self.acceptors[:, 0],
+                                               self.acceptors[:, 1])
+        if self.dim == 1:
+            return self._laplace.evaluate(self.V_nodes, self.acceptors[:, 0])
+        return laplace.potential_3d_slab(self.acceptors, self.electrodes, self.xdim, self.ydim,
+                                         self.res, self.mu, self.static_electrodes)
+
+    def calc_E_constant_V(self):
+        """kmc_dopant_networks.py:832-863."""
+        self.eV_constant = np.asarray(self._eV(), dtype=np.float64)
+        self.comp_constant = np.zeros((self.N,))
+        self.E_constant = self.eV_constant
+        self.site_energies[self.N:] = self.electrodes[:, 3]
+
+    def calc_E_constant_V_comp(self):
+        """kmc_dopant_networks.py:865-899."""
+        self.eV_constant = np.asarray(self._eV(), dtype=np.float64)
+        self.comp_constant = T.calc_comp_constant(self.acceptors, self.donors, self.I_0, self.R)
+        self.E_constant = self.eV_constant + self.comp_constant
+        self.site_energies[self.N:] = self.electrodes[:, 3]
+
+    def load_acceptors(self, acceptors):
+        """kmc_dopant_networks.py:902-930 (sets ab = R, as the reference does)."""
+        self.acceptors = acceptors
+        self.N = self.acceptors.shape[0]
+        self.dim, self.R = T.characteristic_length(self.N, self.xdim, self.ydim, self.zdim)
+        self.ab = self.R
+        self.initialize(V=False, dopant_placement=False)
+
+    def load_donors(self, donors):
+        self.donors = donors
+        self.M = self.donors.shape[0]
+        self.initialize(V=False, dopant_placement=False)
+
+    def saveSelf(self, fileName, rel_path=False):
+        """kmc_dopant_networks.py:943-963 (same .kmc layout)."""
+        if hasattr(self, "current"):
+            self.expected_current = self.current
+        d = {}
+        for key in dir(self):
+            if key.startswith("__"):
+                continue
+            try:
+                attr = getattr(self, key)
+            except AttributeError:
+                continue
+            if isinstance(attr, (list, tuple, int, float, np.ndarray)):
+                d[key] = attr
+        write_kmc(fileName, d)
+
+    def loadSelf(self, fileName, rel_path=False):
+        """kmc_dopant_networks.py:965-978, read without unpickling."""
+        for k, v in read_kmc(fileName).items():
+            if k in ("rng", "V"):
+                continue
+            setattr(self, k, v)
+        self.initialize(dopant_placement=False, charge_placement=False)
+
+    def total_energy(self):
+        return T.total_energy(self.occupation, self.distances, self.eV_constant, self.I_0, self.R)
+
+    @staticmethod
+    def dist(ri, rj):
+        return T.dist(ri, rj)
+
+    # ------------------------------------------------------------ simulation
+    def _layout(self):
+        key = (self.N, self.P, self.ab, self.distances.tobytes(), self.transitions_constant.tobytes())
+        if self._device_layout is None or self._device_layout[0] != key:
+            lay = engine.Layout(self.N, self.P, self.nu, self.kT, self.I_0, self.R, self.distances,
+                                self.transitions_constant)
+            self._device_layout = (key, lay)
+        return self._device_layout[1]
+
+    def _draw_seed(self) -> int:
+        return int(self.rng.randint(0, 2 ** 31 - 1)) * (2 ** 31) + int(self.rng.randint(0, 2 ** 31 - 1))
+
+    def _fp64_hops(self, hops, record):
+        se = np.ascontiguousarray(self.site_energies, dtype=np.float64)
+        occ = np.asarray(self.occupation, dtype=bool).copy()
+        eo = np.zeros(self.P, dtype=np.int64)
+        t, traffic, oit, _, _ = engine.simulate_discrete(
+            self.N, self.P, self.nu, self.kT, self.I_0, self.R, self.distances, self.E_constant,
+            se, self.transitions_constant, occ, eo, int(hops), record=record,
+            seed=self._draw_seed())
+        self.site_energies = se
+        self.occupation[...] = occ
+        return t, eo, traffic, oit
+
+    def python_simulation(self, hops=1E5, prehops=0, record=False):
+        """FP64 dense GPU kernel with the numba loop's semantics (:513-542)."""
+        self.makeSimulation(prehops=prehops, hops=hops, record=record, back_end="fp64")
+
+    simulate_discrete = python_simulation
+
+    def go_simulation(self, hops=1E5, prehops=0, goSpecificFunction="wrapperSimulateRecord",
+                      record=False, prune_threshold=0):
+        """FP32 batched GPU kernel with the Go library's semantics (:473-511)."""
+        if goSpecificFunction not in ("wrapperSimulate", "wrapperSimulateRecord",
+                                      "wrapperSimulateRecordPlus", "wrapperSimulatePruned"):
+            raise _lib.KmcError(_lib.ERR_UNSUPPORTED, f"{goSpecificFunction} is not a hop loop")
+        self.makeSimulation(prehops=prehops, hops=hops, record=record, back_end="fp32",
+                            prune_threshold=prune_threshold if goSpecificFunction ==
+                            "wrapperSimulatePruned" else 0.0)
+
+    def makeSimulation(self, simulateFunction=None, preHopFunction=None, prehops=0, hops=1E5,
+                       record=False, goSpecificFunction=None, prune_threshold=0.0,
+                       back_end="fp64"):
+        """kmc_dopant_networks.py:544-618: prehops (FP64, as the reference always uses numba),
+        reset, hops, then current = electrode_occupation / time."""
+        self.reset()
+        self.reset()
+        if prehops != 0:
+            self._fp64_hops(int(prehops), False)
+            self.reset()
+        hops = int(hops)
+        if back_end == "fp64":
+            t, eo, traffic, oit = self._fp64_hops(hops, record)
+            self.time = t
+            self.electrode_occupation = eo
+            if record:
+                self.traffic = traffic
+                self.average_occupation = [x / self.time for x in oit]
+        else:
+            lay = self._layout() if prune_threshold == 0 else engine.Layout(
+                self.N, self.P, self.nu, self.kT, self.I_0, self.R, self.distances,
+                self.transitions_constant, prune_threshold)
+            es = lay.energy_sets(self.E_constant, self.site_energies[self.N:])
+            # the Go wrappers start from an empty network (simulationWrapper.go:134-141)
+            b = engine.Batch(engine.make_trajs(1, lay.id, es[0], self._draw_seed(), hops,
+                                               np.zeros(self.N, bool)))
+            b.run(precision=_lib.FP32_BATCHED, record=record, post_hop_record=False)
+            b.sync()
+            r = b.results()
+            b.close()
+            _lib.check(int(r.status[0]))
+            self.time = float(r.time[0])
+            self.electrode_occupation = r.eo[0, :self.P].astype(np.int64)
+            self.occupation = engine.bits_to_occupation(r.occ_bits[0], self.N)
+            if record:
+                self.average_occupation = [x / self.time for x in r.occ_time[0, :self.N]]
+        self.current = self.electrode_occupation / self.time
