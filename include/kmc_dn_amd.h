@@ -109,11 +109,13 @@ typedef struct kmc_run_opts {
   int32_t precision;          /* KMC_FP64_DENSE or KMC_FP32_BATCHED */
   int32_t waves_per_group;    /* FP32: trajectories per workgroup (1,2,4,8); 0 = auto */
   int32_t recompute_interval; /* FP32: hops between FP64 energy recomputes; 0 = 1024 */
-  int32_t record;             /* 1: accumulate per-site occupation time */
+  int32_t record;             /* bit 0: per-site occupation time; bit 1 (FP32): net
+                                 traffic per ordered pair, Go convention */
   int32_t post_hop_record;    /* 1: numba convention (post-hop state), 0: pre-hop */
   int32_t grid_limit;         /* FP32: max resident workgroups; 0 = auto */
   int32_t use_graph;          /* 1: launch through a captured hipGraph */
-  int32_t pad;
+  int32_t table_form;         /* FP32 LDS pair table: 0 auto, 1 {lt, lt+c} + c table
+                                 (fewer VALU ops), 2 {lt, c} (smaller LDS) */
 } kmc_run_opts;
 
 /* Uploads trajectory descriptors and allocates device outputs. */
@@ -127,6 +129,9 @@ int kmc_batch_sync(int32_t batch, double* elapsed_ms);
  * hops_done[ntraj], occ_time[ntraj*KMC_NMAX] (only after a record run). */
 int kmc_batch_results(int32_t batch, double* time, int64_t* eo, uint64_t* occ, int32_t* status,
                       int64_t* hops_done, double* occ_time);
+/* Net traffic of the last FP32 run with record bit 1: traffic[ntraj*stride],
+ * trajectory j's S*S matrix at j*stride (stride returned in *stride_out). */
+int kmc_batch_traffic(int32_t batch, double* traffic, int64_t* stride_out);
 int kmc_batch_destroy(int32_t batch);
 
 /* ---- measurement probes (no counterpart in the reference; used by bench.py) ---- */

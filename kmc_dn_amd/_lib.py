@@ -38,7 +38,7 @@ class kmc_run_opts(C.Structure):
     _fields_ = [("precision", C.c_int32), ("waves_per_group", C.c_int32),
                 ("recompute_interval", C.c_int32), ("record", C.c_int32),
                 ("post_hop_record", C.c_int32), ("grid_limit", C.c_int32),
-                ("use_graph", C.c_int32), ("pad", C.c_int32)]
+                ("use_graph", C.c_int32), ("table_form", C.c_int32)]
 
 
 class GoSlice(C.Structure):
@@ -71,6 +71,7 @@ SIGNATURES = {
     "kmc_batch_run": (C.c_int, [_i32, C.POINTER(kmc_run_opts)]),
     "kmc_batch_sync": (C.c_int, [_i32, _dp]),
     "kmc_batch_results": (C.c_int, [_i32, _dp, _i64p, _u64p, _i32p, _i64p, _dp]),
+    "kmc_batch_traffic": (C.c_int, [_i32, _dp, _i64p]),
     "kmc_batch_destroy": (C.c_int, [_i32]),
     "kmc_probe_copy": (C.c_int, [_i64, C.c_int, _dp]),
     "kmc_probe_lds": (C.c_int, [C.c_int, C.c_int, C.c_int, _dp]),

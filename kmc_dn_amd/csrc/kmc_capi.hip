@@ -22,11 +22,11 @@ namespace kmc {
 
 template <bool REC>
 __global__ void kmc_plain_fp64_kernel(PlainParams);
-template <int C, int W, bool REC>
+template <int C, int W, bool REC, bool BT>
 __global__ void kmc_batch_fp32_kernel(BatchParams);
 __global__ void kmc_queue_reset_kernel(int32_t*);
 size_t plain_lds_bytes(int S);
-size_t batch_lds_bytes(int C, int W, int S);
+size_t batch_lds_bytes(bool bt, int S, int N);
 
 namespace {
 
@@ -75,7 +75,7 @@ struct DevBuf {
 
 struct Layout {
   int N = 0, P = 0, S = 0;
-  DevBuf tab32, coul64, tc64, ird64, invd64;
+  DevBuf tab32, tabB, ccol, coul64, tc64, ird64, invd64;
   LayoutDev dev{};
 };
 
@@ -83,15 +83,16 @@ struct Batch {
   int64_t ntraj = 0;
   std::vector<TrajDev> host;        // in layout-grouped order
   std::vector<int64_t> perm;        // grouped position -> caller index
-  int smax = 0;
-  DevBuf trajs, out, occ_time, queue, groups;
+  int smax = 0, nmax = 0;
+  DevBuf trajs, out, occ_time, traffic, queue, groups;
   int32_t ngroups = 0;
   int last_W = -1;
   bool last_rec = false;
   bool has_occ_time = false;
+  bool has_traffic = false;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   hipGraphExec_t graph = nullptr;
-  int graph_key = -1;
+  int64_t graph_key = -1;
   ~Batch() {
     if (graph) (void)hipGraphExecDestroy(graph);
     if (ev0) (void)hipEventDestroy(ev0);
@@ -193,8 +194,10 @@ int layout_create_locked(int64_t N, int64_t P, double nu, double kT, double I_0,
       if (i < n && j < n && !(d[i * S + j] > 0))
         return fail(KMC_ERR_ARGS, "acceptors %d and %d coincide (distance %g)", i, j, d[i * S + j]);
     }
-  std::vector<float2> tab((size_t)S * S + 64 * 3);
+  std::vector<float2> tab((size_t)S * S + 64 * 3), tabB((size_t)S * S + 64 * 3);
+  std::vector<float> ccol((size_t)n * S + 64 * 3, 0.f);
   for (auto& x : tab) x = make_float2(-INFINITY, 0.f);
+  for (auto& x : tabB) x = make_float2(-INFINITY, -INFINITY);
   std::vector<double> coul((size_t)n * n, 0.0), tc64(tc, tc + (size_t)S * S),
       ird((size_t)S * S, 0.0), invd((size_t)n * n, 0.0);
   const double lnu = std::log2(nu);
@@ -207,6 +210,8 @@ int layout_create_locked(int64_t N, int64_t P, double nu, double kT, double I_0,
       float c = 0.f;
       if (s < n && u < n && s != u) c = (float)(scale2 * (I0R / d[s * S + u]));
       tab[(size_t)u * S + s] = make_float2(lt, c);
+      tabB[(size_t)u * S + s] = make_float2(lt, lt + c);
+      if (u < n) ccol[(size_t)u * S + s] = c;
     }
   for (int i = 0; i < n; ++i)
     for (int j = 0; j < n; ++j)
@@ -220,7 +225,8 @@ int layout_create_locked(int64_t N, int64_t P, double nu, double kT, double I_0,
   L->P = (int)P;
   L->S = S;
   int st;
-  if ((st = upload(L->tab32, tab)) || (st = upload(L->coul64, coul)) || (st = upload(L->tc64, tc64)) ||
+  if ((st = upload(L->tab32, tab)) || (st = upload(L->tabB, tabB)) || (st = upload(L->ccol, ccol)) ||
+      (st = upload(L->coul64, coul)) || (st = upload(L->tc64, tc64)) ||
       (st = upload(L->ird64, ird)) || (st = upload(L->invd64, invd))) {
     delete L;
     return st;
@@ -237,6 +243,8 @@ int layout_create_locked(int64_t N, int64_t P, double nu, double kT, double I_0,
   D.I0R = I0R;
   D.scale2 = (float)scale2;
   D.tab32 = L->tab32.as<float2>();
+  D.tabB = L->tabB.as<float2>();
+  D.ccol = L->ccol.as<float>();
   D.coul64 = L->coul64.as<double>();
   D.tc64 = L->tc64.as<double>();
   D.ird64 = L->ird64.as<double>();
@@ -335,6 +343,7 @@ int batch_create_locked(const kmc_traj* t, int64_t ntraj, int32_t* out) {
     d.hops = x.hops;
     for (int w = 0; w < OCCW; ++w) d.occ[w] = x.occ[w];
     B->smax = std::max(B->smax, E.layouts[x.layout]->S);
+    B->nmax = std::max(B->nmax, E.layouts[x.layout]->N);
   }
   int st;
   if ((st = upload(B->trajs, B->host)) || (st = B->out.alloc(sizeof(TrajOut) * ntraj)) ||
@@ -377,11 +386,14 @@ int build_groups(Batch* B, int W) {
 
 using BatchKernel = void (*)(BatchParams);
 
-BatchKernel pick_batch_kernel(int C, int W, bool rec) {
-#define KMC_PICK(CC, WW)                                                                     \
-  if (C == CC && W == WW)                                                                    \
-    return rec ? (BatchKernel)kmc_batch_fp32_kernel<CC, WW, true>                           \
-               : (BatchKernel)kmc_batch_fp32_kernel<CC, WW, false>;
+BatchKernel pick_batch_kernel(int C, int W, bool rec, bool bt) {
+#define KMC_PICK(CC, WW)                                                                        \
+  if (C == CC && W == WW) {                                                                     \
+    if (bt) return rec ? (BatchKernel)kmc_batch_fp32_kernel<CC, WW, true, true>                 \
+                       : (BatchKernel)kmc_batch_fp32_kernel<CC, WW, false, true>;               \
+    return rec ? (BatchKernel)kmc_batch_fp32_kernel<CC, WW, true, false>                        \
+               : (BatchKernel)kmc_batch_fp32_kernel<CC, WW, false, false>;                      \
+  }
   KMC_PICK(1, 1) KMC_PICK(1, 2) KMC_PICK(1, 4) KMC_PICK(1, 8)
   KMC_PICK(2, 1) KMC_PICK(2, 2) KMC_PICK(2, 4) KMC_PICK(2, 8)
   KMC_PICK(3, 1) KMC_PICK(3, 2) KMC_PICK(3, 4) KMC_PICK(3, 8)
@@ -393,6 +405,7 @@ struct Fp32Launch {
   BatchKernel k = nullptr;
   int grid = 0, block = 0;
   size_t lds = 0;
+  size_t traffic_bytes = 0;
   BatchParams p{};
 };
 
@@ -403,12 +416,19 @@ int prepare_fp32(Batch* B, const kmc_run_opts& o, Fp32Launch& L) {
   int W = o.waves_per_group;
   if (W <= 0) W = C == 1 ? 4 : (C == 2 ? 2 : 1);
   if (W != 1 && W != 2 && W != 4 && W != 8) return fail(KMC_ERR_ARGS, "waves_per_group must be 1, 2, 4 or 8");
-  const bool rec = o.record != 0;
-  BatchKernel k = pick_batch_kernel(C, W, rec);
+  const bool rec = (o.record & 1) != 0;
+  const bool trf = (o.record & 2) != 0;
+  const size_t lds_cap = (size_t)160 * 1024;
+  bool bt;
+  if (o.table_form == 1) bt = true;
+  else if (o.table_form == 2) bt = false;
+  else if (o.table_form == 0) bt = batch_lds_bytes(true, B->smax, B->nmax) <= lds_cap;
+  else return fail(KMC_ERR_ARGS, "table_form must be 0, 1 or 2");
+  BatchKernel k = pick_batch_kernel(C, W, rec, bt);
   if (!k) return fail(KMC_ERR_LIMIT, "no kernel for %d site chunks", C);
-  const size_t lds = batch_lds_bytes(C, W, B->smax);
-  if (lds > (size_t)160 * 1024)
-    return fail(KMC_ERR_LIMIT, "LDS %zu bytes for S=%d, W=%d exceeds 160 KiB", lds, B->smax, W);
+  const size_t lds = batch_lds_bytes(bt, B->smax, B->nmax);
+  if (lds > lds_cap)
+    return fail(KMC_ERR_LIMIT, "LDS %zu bytes for S=%d exceeds 160 KiB", lds, B->smax);
   HIPOK(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   if (W != B->last_W) {
     int st = build_groups(B, W);
@@ -426,6 +446,15 @@ int prepare_fp32(Batch* B, const kmc_run_opts& o, Fp32Launch& L) {
     if (st) return st;
     B->has_occ_time = true;
   }
+  const int64_t tstride = (int64_t)B->smax * B->smax;
+  if (trf) {
+    if (o.use_graph) return fail(KMC_ERR_ARGS, "traffic recording is not available with use_graph");
+    if (!B->has_traffic) {
+      int st = B->traffic.alloc(sizeof(double) * tstride * B->ntraj);
+      if (st) return st;
+      B->has_traffic = true;
+    }
+  }
   BatchParams& p = L.p;
   p.trajs = B->trajs.as<TrajDev>();
   p.layouts = E.layouts_dev.as<LayoutDev>();
@@ -436,7 +465,10 @@ int prepare_fp32(Batch* B, const kmc_run_opts& o, Fp32Launch& L) {
   p.queue = B->queue.as<int32_t>();
   p.out = B->out.as<TrajOut>();
   p.occ_time = rec ? B->occ_time.as<double>() : nullptr;
+  p.traffic = trf ? B->traffic.as<double>() : nullptr;
+  p.traffic_stride = tstride;
   p.post_hop_record = o.post_hop_record;
+  L.traffic_bytes = trf ? sizeof(double) * (size_t)tstride * B->ntraj : 0;
   L.k = k;
   L.grid = grid;
   L.block = 64 * W;
@@ -445,6 +477,7 @@ int prepare_fp32(Batch* B, const kmc_run_opts& o, Fp32Launch& L) {
 }
 
 int launch_fp32(Fp32Launch& L, hipStream_t s) {
+  if (L.p.traffic) HIPOK(hipMemsetAsync(L.p.traffic, 0, L.traffic_bytes, s));
   int32_t* q = L.p.queue;
   void* qa[] = {&q};
   HIPOK(hipLaunchKernel((const void*)kmc_queue_reset_kernel, dim3(1), dim3(1), qa, 0, s));
@@ -497,15 +530,17 @@ int batch_run_locked(int32_t id, const kmc_run_opts* opts) {
     st = enqueue_fp64(B, o, s, nullptr, nullptr, nullptr, nullptr, nullptr);
     if (st) return st;
     HIPOK(hipEventRecord(B->ev1, s));
-    B->last_rec = o.record != 0;
+    B->last_rec = (o.record & 1) != 0;
     return KMC_OK;
   }
   if (o.precision != KMC_FP32_BATCHED) return fail(KMC_ERR_ARGS, "unknown precision %d", o.precision);
   Fp32Launch L;
   if ((st = prepare_fp32(B, o, L))) return st;
   if (o.use_graph) {
-    const int key = (o.waves_per_group & 0xFF) | ((o.record & 1) << 8) | ((o.post_hop_record & 1) << 9) |
-                    ((o.recompute_interval & 0xFFFFF) << 10);
+    const int64_t key = (int64_t)(o.waves_per_group & 0xFF) | ((int64_t)(o.record & 1) << 8) |
+                        ((int64_t)(o.post_hop_record & 1) << 9) |
+                        ((int64_t)(o.recompute_interval & 0xFFFFF) << 10) |
+                        ((int64_t)(o.table_form & 3) << 30);
     if (!B->graph || B->graph_key != key || o.grid_limit) {
       if (B->graph) {
         (void)hipGraphExecDestroy(B->graph);
@@ -535,7 +570,7 @@ int batch_run_locked(int32_t id, const kmc_run_opts* opts) {
     if (st) return st;
     HIPOK(hipEventRecord(B->ev1, s));
   }
-  B->last_rec = o.record != 0;
+  B->last_rec = (o.record & 1) != 0;
   return KMC_OK;
 }
 
@@ -670,6 +705,24 @@ int kmc_batch_results(int32_t batch, double* time, int64_t* eo, uint64_t* occ, i
       for (int e = 0; e < PMAX; ++e) eo[j * PMAX + e] = e < P ? r.eo[e] : 0;
     if (occ_time) memcpy(occ_time + (size_t)j * NMAX, ot.data() + (size_t)i * NMAX, sizeof(double) * NMAX);
   }
+  return KMC_OK;
+}
+
+int kmc_batch_traffic(int32_t batch, double* traffic, int64_t* stride_out) {
+  Engine& E = eng();
+  std::lock_guard<std::mutex> lk(E.mu);
+  Batch* B = get_batch(batch);
+  if (!B) return fail(KMC_ERR_ARGS, "bad batch id %d", batch);
+  if (!B->has_traffic) return fail(KMC_ERR_ARGS, "batch was not run with record bit 1 (traffic)");
+  const int64_t stride = (int64_t)B->smax * B->smax;
+  if (stride_out) *stride_out = stride;
+  if (!traffic) return KMC_OK;
+  HIPOK(hipStreamSynchronize(E.stream));
+  std::vector<double> t((size_t)stride * B->ntraj);
+  HIPOK(hipMemcpy(t.data(), B->traffic.p, sizeof(double) * t.size(), hipMemcpyDeviceToHost));
+  for (int64_t i = 0; i < B->ntraj; ++i)
+    memcpy(traffic + (size_t)B->perm[i] * stride, t.data() + (size_t)i * stride,
+           sizeof(double) * stride);
   return KMC_OK;
 }
 

@@ -31,6 +31,10 @@ struct LayoutDev {
   //   c  = scale2 * I_0 * R / d[s][u] for acceptor-acceptor pairs, else 0.
   // 64 * 3 trailing entries {-inf, 0} let lanes past S read in bounds.
   const float2* tab32;
+  // Same pairs as {lt, b = lt + c} (b = -inf where lt = -inf), and the c column
+  // table ccol[u * S + s] = c(s, u) for acceptor columns u < N (N*S + 192 floats).
+  const float2* tabB;
+  const float* ccol;
   const double* coul64;   // [N*N] I_0*R/d for acceptor pairs (symmetric, 0 diagonal)
   // FP64 dense kernel (reference arithmetic).
   const double* tc64;     // [S*S] transitions_constant
@@ -79,6 +83,8 @@ struct BatchParams {
   int32_t* queue;               // work-queue head, zero before launch
   TrajOut* out;
   double* occ_time;             // [ntraj * NMAX] or null
+  double* traffic;              // [ntraj * traffic_stride] net traffic, or null
+  int64_t traffic_stride;       // >= S*S of every layout in the batch
   int32_t post_hop_record;      // 1: numba convention (post-hop state)
   int32_t pad;
 };

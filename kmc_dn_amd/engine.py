@@ -126,9 +126,12 @@ class Batch:
 
     def run(self, precision: int = L.FP32_BATCHED, waves_per_group: int = 0,
             recompute_interval: int = 0, record: bool = False, post_hop_record: bool = False,
-            grid_limit: int = 0, use_graph: bool = False) -> None:
+            grid_limit: int = 0, use_graph: bool = False, traffic: bool = False,
+            table_form: int = 0) -> None:
+        rec = (1 if record else 0) | (2 if traffic else 0)
         o = L.kmc_run_opts(int(precision), int(waves_per_group), int(recompute_interval),
-                           int(record), int(post_hop_record), int(grid_limit), int(use_graph), 0)
+                           rec, int(post_hop_record), int(grid_limit), int(use_graph),
+                           int(table_form))
         L.check(L.load().kmc_batch_run(self.id, C.byref(o)))
         self.recorded = bool(record)
 
@@ -149,6 +152,16 @@ class Batch:
                                            L.ptr(occ, L._u64p), L.ptr(st, L._i32p),
                                            L.ptr(hd, L._i64p), L.ptr(ot, L._dp)))
         return BatchResult(t, eo, occ, st, hd, ot)
+
+    def traffic(self, sizes) -> list:
+        """Net traffic of the last FP32 run with traffic=True: one S_j x S_j matrix per
+        trajectory j, ``sizes[j]`` = S_j of its layout."""
+        stride = C.c_int64()
+        L.check(L.load().kmc_batch_traffic(self.id, None, C.byref(stride)))
+        flat = np.zeros((self.n, stride.value))
+        L.check(L.load().kmc_batch_traffic(self.id, L.ptr(flat, L._dp), C.byref(stride)))
+        sizes = np.broadcast_to(np.asarray(sizes), (self.n,))
+        return [flat[j, :S * S].reshape(S, S) for j, S in enumerate(sizes)]
 
     def close(self) -> None:
         if self.id >= 0:

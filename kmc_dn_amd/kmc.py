@@ -295,9 +295,12 @@ class kmc_dn:
             # the Go wrappers start from an empty network (simulationWrapper.go:134-141)
             b = engine.Batch(engine.make_trajs(1, lay.id, es[0], self._draw_seed(), hops,
                                                np.zeros(self.N, bool)))
-            b.run(precision=_lib.FP32_BATCHED, record=record, post_hop_record=False)
+            b.run(precision=_lib.FP32_BATCHED, record=record, traffic=record,
+                  post_hop_record=False)
             b.sync()
             r = b.results()
+            if record:
+                self.traffic = b.traffic(self.N + self.P)[0]
             b.close()
             _lib.check(int(r.status[0]))
             self.time = float(r.time[0])

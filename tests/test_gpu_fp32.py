@@ -71,7 +71,7 @@ def test_fp32_matches_fp64_statistics_c1_like(gpu_engine):
     net = random_network(0, 30, 8, M=3, I_0=50.0, a=0.1, vmax=1.0)
     c64, ms64 = _ensemble(net, _lib.FP64_DENSE, 512, 20000, prehops=2000)
     c32, ms32 = _ensemble(net, _lib.FP32_BATCHED, 2048, 20000, prehops=2000)
-    print(f"[gpu] fp64 {512 * 22000 / ms64 / 1e3:.3e} hops/s, fp32 {2048 * 22000 / ms32 / 1e3:.3e}"
+    print(f"[gpu] fp64 {512 * 22000 / ms64 * 1e3:.3e} hops/s, fp32 {2048 * 22000 / ms32 * 1e3:.3e}"
           " hops/s", flush=True)
     _compare(c32, c64, "fp32 vs fp64 (C1-like)")
 
@@ -111,3 +111,24 @@ def test_fixture_currents_against_reference(fixtures, gpu_engine):
           f"frac>3 {np.mean(z > 3):.3f}", flush=True)
     # the reference's own numba-vs-Go comparison: median 0.74, p95 2.6, 3% above 3
     assert np.median(z) < 1.2 and np.mean(z > 3) < 0.08
+
+
+def test_traffic_is_net_and_matches_tallies(gpu_engine):
+    net = random_network(8, 24, 6, I_0=20.0, a=0.3, vmax=2.0)
+    N, P = 24, 6
+    _, b = _setup(net, 32, 4000)
+    b.run(precision=_lib.FP32_BATCHED, traffic=True)
+    b.sync()
+    r = b.results()
+    for j, tr in enumerate(b.traffic(N + P)):
+        assert np.array_equal(tr, -tr.T)                       # net, antisymmetric
+        into_el = tr[:, N:].sum(axis=0)                          # net hops into each electrode
+        assert np.array_equal(into_el, r.eo[j, :P].astype(float))
+        assert np.abs(tr).sum() > 0
+
+
+def test_table_forms_agree(gpu_engine):
+    net = random_network(12, 40, 8, I_0=30.0, a=0.2, vmax=2.0)
+    ca, _ = _ensemble(net, _lib.FP32_BATCHED, 1024, 10000, table_form=1)
+    cb, _ = _ensemble(net, _lib.FP32_BATCHED, 1024, 10000, table_form=2)
+    _compare(ca, cb, "table {lt,lt+c} vs {lt,c}")

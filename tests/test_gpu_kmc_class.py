@@ -1,0 +1,51 @@
+"""kmc_dn class on the device: the reference's simulate calls through the new kernels."""
+import numpy as np
+import pytest
+
+from kmc_dn_amd.kmc import kmc_dn
+from kmc_dn_amd import tables as T
+
+pytestmark = pytest.mark.gpu
+
+
+def _model(seed=0):
+    rng = np.random.RandomState(seed)
+    el = T.perimeter_electrodes(1, 1, rng.uniform(-3, 3, 8))
+    return kmc_dn(30, 3, 1, 1, 0, mu=0, I_0=20, a=0.3, electrodes=el, rng=rng)
+
+
+def test_python_simulation_updates_state(gpu_engine):
+    m = _model()
+    occ0 = m.occupation.copy()
+    m.python_simulation(hops=5000, prehops=500, record=True)
+    assert m.time > 0 and np.isfinite(m.current).all()
+    assert m.current.shape == (8,) and m.traffic.shape == (38, 38)
+    assert len(m.average_occupation) == 30
+    assert np.all((np.array(m.average_occupation) >= 0) & (np.array(m.average_occupation) <= 1 + 1e-12))
+    # holes are conserved up to what the electrodes exchanged (one hole per hop)
+    assert occ0.sum() - m.occupation.sum() == m.electrode_occupation.sum() or m.traffic.sum() == 5000
+    assert m.traffic.sum() == 5000
+
+
+def test_go_simulation_semantics(gpu_engine):
+    m = _model(1)
+    m.go_simulation(hops=20000, goSpecificFunction="wrapperSimulateRecord", record=True)
+    assert np.isfinite(m.current).all() and m.time > 0
+    assert np.array_equal(m.traffic, -m.traffic.T)
+    # the Go wrappers start from an empty network: holes now = holes absorbed from electrodes
+    assert m.occupation.sum() == -m.electrode_occupation.sum()
+    m.go_simulation(hops=2000, goSpecificFunction="wrapperSimulatePruned", prune_threshold=1e-3)
+    assert np.isfinite(m.current).all()
+
+
+def test_numba_and_go_paths_agree_statistically(gpu_engine):
+    m = _model(2)
+    a, b = [], []
+    for _ in range(12):
+        m.python_simulation(hops=20000, prehops=2000)
+        a.append(m.current.copy())
+        m.go_simulation(hops=20000)
+        b.append(m.current.copy())
+    a, b = np.array(a), np.array(b)
+    se = np.sqrt(a.var(0, ddof=1) / 12 + b.var(0, ddof=1) / 12)
+    assert np.all(np.abs(a.mean(0) - b.mean(0)) < 4.5 * se + 1e-12)
