@@ -22,9 +22,8 @@ def test_python_simulation_updates_state(gpu_engine):
     assert m.current.shape == (8,) and m.traffic.shape == (38, 38)
     assert len(m.average_occupation) == 30
     assert np.all((np.array(m.average_occupation) >= 0) & (np.array(m.average_occupation) <= 1 + 1e-12))
-    # holes are conserved up to what the electrodes exchanged (one hole per hop)
-    assert occ0.sum() - m.occupation.sum() == m.electrode_occupation.sum() or m.traffic.sum() == 5000
-    assert m.traffic.sum() == 5000
+    assert m.traffic.sum() == 5000          # gross counts, one per hop (numba convention)
+    assert occ0.dtype == bool and m.occupation.dtype == bool
 
 
 def test_go_simulation_semantics(gpu_engine):
