@@ -92,6 +92,29 @@ def cpu_baselines(ens, seconds: float) -> tuple[dict, dict]:
     return numba, go
 
 
+def iv_leg(lay, args) -> dict:
+    """IV sweep: electrode 0 over linspace(-2, 2, n), the others held, R replicas each,
+    all points x replicas in one launch on the shared layout."""
+    from kmc_dn_amd import _lib, iv, workloads
+    sw = workloads.iv_sweep(0, args.iv_points, args.iv_replicas, args.iv_hops)
+    b = iv.iv_batch(lay, sw.E_sets, sw.V_sets, sw.replicas, sw.hops, sw.net.occupation)
+    t0 = time.perf_counter()
+    b.run(precision=_lib.FP32_BATCHED)
+    ms = b.sync()
+    wall = time.perf_counter() - t0
+    r = b.results()
+    assert (r.status == 0).all()
+    mean, se = iv.currents_from(r, args.iv_points, args.iv_replicas, sw.net.P)
+    hops = args.iv_points * args.iv_replicas * args.iv_hops
+    b.close()
+    return {"value": hops / wall, "unit": "hops/s", "kernel_ms": ms,
+            "trajectories": args.iv_points * args.iv_replicas, "hops_per_traj": args.iv_hops,
+            "workload": f"electrode 0 swept over linspace(-2,2,{args.iv_points}), others held; "
+                        f"{args.iv_replicas} replicas per point; energy sets by Laplace-basis "
+                        "superposition",
+            "current_e0_first_last": [float(mean[0, 0]), float(mean[0, -1])]}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -105,6 +128,10 @@ def main() -> int:
     ap.add_argument("--plain-hops", type=int, default=2000)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--iv-points", type=int, default=1024)
+    ap.add_argument("--iv-replicas", type=int, default=64)
+    ap.add_argument("--iv-hops", type=int, default=200_000)
+    ap.add_argument("--no-iv", action="store_true")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -194,6 +221,8 @@ def main() -> int:
         "speedup_vs_plain_fp64": value / plain_value,
         "mean_current": [float(x) for x in cur.mean(0)],
     }
+    if not args.no_iv:
+        out["iv_sweep"] = iv_leg(lay, args)
     if not args.no_cpu_baseline:
         numba, go = cpu_baselines(ens, args.cpu_seconds)
         out["cpu_baseline"] = numba

@@ -20,12 +20,12 @@
 
 namespace kmc {
 
-template <bool REC>
+template <bool REC, bool STAGE>
 __global__ void kmc_plain_fp64_kernel(PlainParams);
 template <int C, int W, bool REC, bool BT>
 __global__ void kmc_batch_fp32_kernel(BatchParams);
 __global__ void kmc_queue_reset_kernel(int32_t*);
-size_t plain_lds_bytes(int S);
+size_t plain_lds_bytes(int S, int N, bool stage);
 size_t batch_lds_bytes(bool bt, int S, int N);
 
 namespace {
@@ -497,8 +497,12 @@ int enqueue_fp64(Batch* B, const kmc_run_opts& o, hipStream_t s, const double* u
     if (st) return st;
     B->has_occ_time = true;
   }
-  const size_t lds = plain_lds_bytes(B->smax);
-  auto k = rec ? kmc_plain_fp64_kernel<true> : kmc_plain_fp64_kernel<false>;
+  const size_t cap = (size_t)160 * 1024;
+  const bool stage = plain_lds_bytes(B->smax, B->nmax, true) <= cap;
+  const size_t lds = plain_lds_bytes(B->smax, B->nmax, stage);
+  if (lds > cap) return fail(KMC_ERR_LIMIT, "FP64 kernel needs %zu bytes of LDS", lds);
+  void (*k)(PlainParams) = rec ? (stage ? kmc_plain_fp64_kernel<true, true> : kmc_plain_fp64_kernel<true, false>)
+                               : (stage ? kmc_plain_fp64_kernel<false, true> : kmc_plain_fp64_kernel<false, false>);
   HIPOK(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   PlainParams p{};
   p.trajs = B->trajs.as<TrajDev>();

@@ -130,8 +130,23 @@ __device__ __forceinline__ float wave_scan_f32(float x) {
 __device__ __forceinline__ float fast_exp2(float x) { return __builtin_amdgcn_exp2f(x); }
 __device__ __forceinline__ float fast_log2(float x) { return __builtin_amdgcn_logf(x); }
 
-__device__ __forceinline__ bool occ_bit(const uint64_t* occ, int i) {
-  return (occ[i >> 6] >> (i & 63)) & 1ull;
-}
+// Occupation bitmask of up to 256 acceptors as four named words.  Lane-varying
+// bit tests select among SSA values; a private uint64_t[4] indexed that way is
+// turned into scratch-memory loads by the compiler.
+struct Occ4 {
+  uint64_t a, b, c, d;
+  __device__ __forceinline__ uint64_t word(int w) const {
+    return w == 0 ? a : (w == 1 ? b : (w == 2 ? c : d));
+  }
+  __device__ __forceinline__ bool test(int i) const { return (word(i >> 6) >> (i & 63)) & 1ull; }
+  __device__ __forceinline__ void assign(int i, bool v) {
+    const uint64_t bit = 1ull << (i & 63);
+    const int w = i >> 6;
+    if (w == 0) a = v ? (a | bit) : (a & ~bit);
+    else if (w == 1) b = v ? (b | bit) : (b & ~bit);
+    else if (w == 2) c = v ? (c | bit) : (c & ~bit);
+    else d = v ? (d | bit) : (d & ~bit);
+  }
+};
 
 }  // namespace kmc
