@@ -90,6 +90,9 @@ int kmc_layout_create(int64_t N, int64_t P, double nu, double kT, double I_0, do
                       const double* distances, const double* transitions_constant,
                       double prune_threshold, int32_t* layout_out);
 
+/* Frees a layout's tables and every energy set created on it (ids may be reused). */
+int kmc_layout_destroy(int32_t layout);
+
 /* Uploads `count` energy sets of a layout: E_constant count*N, V count*P
  * (electrode voltages, i.e. site_energies[N:]).  Ids first..first+count-1. */
 int kmc_eset_create(int32_t layout, int64_t count, const double* E_constant, const double* V,
@@ -133,6 +136,64 @@ int kmc_batch_results(int32_t batch, double* time, int64_t* eo, uint64_t* occ, i
  * trajectory j's S*S matrix at j*stride (stride returned in *stride_out). */
 int kmc_batch_traffic(int32_t batch, double* traffic, int64_t* stride_out);
 int kmc_batch_destroy(int32_t batch);
+
+/* ---- exact-name exports of the reference's cgo library ----
+ * Same symbols and by-value GoSlice signatures as goSimulation/simulationWrapper.go,
+ * so goSimulation/pythonBind.py and parrallelSimulationBind.py can load this
+ * library as libSimulation.so.  They return NaN (time) or -1 on error; see
+ * kmc_last_error(). */
+typedef struct kmc_GoSlice {
+  double* data;
+  int64_t len;
+  int64_t cap;
+} kmc_GoSlice;
+
+/* simulationWrapper.go:83-96 (simulate, empty start, FP32) */
+double wrapperSimulate(int64_t NSites, int64_t NElectrodes, double nu, double kT, double I_0,
+                       double R, kmc_GoSlice occupation, kmc_GoSlice distances,
+                       kmc_GoSlice E_constant, kmc_GoSlice transitions_constant,
+                       kmc_GoSlice electrode_occupation, kmc_GoSlice site_energies,
+                       int64_t hops, uint8_t record, kmc_GoSlice traffic,
+                       kmc_GoSlice average_occupation);
+/* simulationWrapper.go:98-110 (pair list pruned at prune_threshold * max tc) */
+double wrapperSimulatePruned(int64_t NSites, int64_t NElectrodes, double prune_threshold,
+                             double nu, double kT, double I_0, double R, kmc_GoSlice occupation,
+                             kmc_GoSlice distances, kmc_GoSlice E_constant,
+                             kmc_GoSlice transitions_constant, kmc_GoSlice electrode_occupation,
+                             kmc_GoSlice site_energies, int64_t hops, uint8_t record,
+                             kmc_GoSlice traffic, kmc_GoSlice average_occupation);
+/* simulationWrapper.go:127-147 */
+double wrapperSimulateRecord(int64_t NSites, int64_t NElectrodes, double nu, double kT,
+                             double I_0, double R, kmc_GoSlice occupation, kmc_GoSlice distances,
+                             kmc_GoSlice E_constant, kmc_GoSlice transitions_constant,
+                             kmc_GoSlice electrode_occupation, kmc_GoSlice site_energies,
+                             int64_t hops, uint8_t record, kmc_GoSlice traffic,
+                             kmc_GoSlice average_occupation);
+/* simulationWrapper.go:149-169 (records nothing) */
+double wrapperSimulateRecordPlus(int64_t NSites, int64_t NElectrodes, double nu, double kT,
+                                 double I_0, double R, kmc_GoSlice occupation,
+                                 kmc_GoSlice distances, kmc_GoSlice E_constant,
+                                 kmc_GoSlice transitions_constant,
+                                 kmc_GoSlice electrode_occupation, kmc_GoSlice site_energies,
+                                 int64_t hops, uint8_t record, kmc_GoSlice traffic,
+                                 kmc_GoSlice average_occupation);
+/* simulationWrapper.go:112-125, :218-233, :171-216 -- not hop loops: return NaN */
+double wrapperSimulateCombined(int64_t, int64_t, double, double, double, double, kmc_GoSlice,
+                               kmc_GoSlice, kmc_GoSlice, kmc_GoSlice, kmc_GoSlice, kmc_GoSlice,
+                               int64_t, uint8_t, kmc_GoSlice, kmc_GoSlice);
+double wrapperSimulateProbability(int64_t, int64_t, double, double, double, double, kmc_GoSlice,
+                                  kmc_GoSlice, kmc_GoSlice, kmc_GoSlice, kmc_GoSlice, kmc_GoSlice,
+                                  int64_t, uint8_t, kmc_GoSlice, kmc_GoSlice);
+double analyzeStateOverlap(int64_t, int64_t, double, double, double, double, kmc_GoSlice,
+                           kmc_GoSlice, kmc_GoSlice, kmc_GoSlice, kmc_GoSlice, kmc_GoSlice, int64_t,
+                           uint8_t, kmc_GoSlice, kmc_GoSlice);
+/* simulationWrapper.go:274-315: K networks in one call, all in one batch */
+int64_t parallelSimulations(kmc_GoSlice NSites, kmc_GoSlice NElectrodes, kmc_GoSlice nu,
+                            kmc_GoSlice kT, kmc_GoSlice I_0, kmc_GoSlice R,
+                            kmc_GoSlice occupation, kmc_GoSlice distances,
+                            kmc_GoSlice E_constant, kmc_GoSlice transitions_constant,
+                            kmc_GoSlice electrode_occupation, kmc_GoSlice hops, kmc_GoSlice time,
+                            kmc_GoSlice site_energies);
 
 /* ---- measurement probes (no counterpart in the reference; used by bench.py) ---- */
 

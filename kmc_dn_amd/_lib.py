@@ -66,6 +66,7 @@ SIGNATURES = {
         _i64p, _i64, C.c_int, _dp, _dp, _dp, C.c_uint64, C.c_uint64, _i32p, _dp, _dp]),
     "kmc_layout_create": (C.c_int, [_i64, _i64, C.c_double, C.c_double, C.c_double, C.c_double,
                                     _dp, _dp, C.c_double, _i32p]),
+    "kmc_layout_destroy": (C.c_int, [_i32]),
     "kmc_eset_create": (C.c_int, [_i32, _i64, _dp, _dp, _i32p]),
     "kmc_batch_create": (C.c_int, [C.POINTER(kmc_traj), _i64, _i32p]),
     "kmc_batch_run": (C.c_int, [_i32, C.POINTER(kmc_run_opts)]),

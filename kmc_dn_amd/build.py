@@ -32,6 +32,7 @@ SOURCES = {
     "kmc_batch.hip": [],
     "kmc_capi.hip": [],
     "kmc_probe.hip": [],
+    "kmc_goshim.cpp": [],
 }
 
 
@@ -47,7 +48,7 @@ def build(verbose: bool = False) -> str:
     with tempfile.TemporaryDirectory(prefix="kmc_build_") as tmp:
         objs = []
         for src, extra in SOURCES.items():
-            obj = os.path.join(tmp, src.replace(".hip", ".o"))
+            obj = os.path.join(tmp, os.path.splitext(src)[0] + ".o")
             cmd = [cc, *COMMON, *extra, "-c", os.path.join(CSRC, src), "-o", obj]
             if verbose:
                 print(" ".join(cmd), flush=True)

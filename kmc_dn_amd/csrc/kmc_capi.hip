@@ -75,6 +75,11 @@ struct DevBuf {
 
 struct Layout {
   int N = 0, P = 0, S = 0;
+  std::vector<DevBuf*> eset_bufs;   // energy sets created on this layout
+  std::vector<int32_t> eset_ids;
+  ~Layout() {
+    for (DevBuf* b : eset_bufs) delete b;
+  }
   DevBuf tab32, tabB, ccol, coul64, tc64, ird64, invd64;
   LayoutDev dev{};
 };
@@ -111,7 +116,6 @@ struct Engine {
   std::vector<Layout*> layouts;
   std::vector<EsetDev> esets_host;
   std::vector<int> eset_layout;
-  std::vector<DevBuf*> eset_bufs;
   DevBuf layouts_dev, esets_dev;
   std::vector<Batch*> batches;
   bool layouts_dirty = true, esets_dirty = true;
@@ -250,9 +254,19 @@ int layout_create_locked(int64_t N, int64_t P, double nu, double kT, double I_0,
   D.ird64 = L->ird64.as<double>();
   D.invd64 = L->invd64.as<double>();
   Engine& E = eng();
-  E.layouts.push_back(L);
+  int32_t id = -1;
+  for (size_t i = 0; i < E.layouts.size(); ++i)
+    if (!E.layouts[i]) {
+      id = (int32_t)i;
+      break;
+    }
+  if (id < 0) {
+    id = (int32_t)E.layouts.size();
+    E.layouts.push_back(nullptr);
+  }
+  E.layouts[id] = L;
   E.layouts_dirty = true;
-  *out = (int32_t)(E.layouts.size() - 1);
+  *out = id;
   return KMC_OK;
 }
 
@@ -262,7 +276,7 @@ int eset_create_locked(int32_t layout, int64_t count, const double* Ec, const do
   if (layout < 0 || layout >= (int)E.layouts.size() || !E.layouts[layout])
     return fail(KMC_ERR_ARGS, "bad layout id %d", layout);
   if (count <= 0 || !first) return fail(KMC_ERR_ARGS, "bad energy-set count");
-  const Layout* L = E.layouts[layout];
+  Layout* L = E.layouts[layout];
   const size_t nE = (size_t)count * L->N, nV = (size_t)count * L->P;
   if ((nE && !Ec) || (nV && !V)) return fail(KMC_ERR_ARGS, "null E_constant or V");
   for (size_t i = 0; i < nE; ++i)
@@ -277,16 +291,45 @@ int eset_create_locked(int32_t layout, int64_t count, const double* Ec, const do
   }
   if (nE) HIPOK(hipMemcpy(b->p, Ec, nE * sizeof(double), hipMemcpyHostToDevice));
   if (nV) HIPOK(hipMemcpy(b->as<double>() + nE, V, nV * sizeof(double), hipMemcpyHostToDevice));
-  *first = (int32_t)E.esets_host.size();
+  int32_t base = -1;
+  if (count == 1)
+    for (size_t i = 0; i < E.eset_layout.size(); ++i)
+      if (E.eset_layout[i] < 0) {
+        base = (int32_t)i;
+        break;
+      }
+  if (base < 0) {
+    base = (int32_t)E.esets_host.size();
+    E.esets_host.resize(E.esets_host.size() + count);
+    E.eset_layout.resize(E.eset_layout.size() + count, -1);
+  }
+  *first = base;
   for (int64_t i = 0; i < count; ++i) {
     EsetDev e;
     e.E = b->as<double>() + (size_t)i * L->N;
     e.V = b->as<double>() + nE + (size_t)i * L->P;
-    E.esets_host.push_back(e);
-    E.eset_layout.push_back(layout);
+    L->eset_ids.push_back(base + (int32_t)i);
+    E.esets_host[base + i] = e;
+    E.eset_layout[base + i] = layout;
   }
-  E.eset_bufs.push_back(b);
+  L->eset_bufs.push_back(b);
   E.esets_dirty = true;
+  return KMC_OK;
+}
+
+int layout_destroy_locked(int32_t layout) {
+  Engine& E = eng();
+  if (layout < 0 || layout >= (int)E.layouts.size() || !E.layouts[layout])
+    return fail(KMC_ERR_ARGS, "bad layout id %d", layout);
+  (void)hipStreamSynchronize(E.stream);
+  Layout* L = E.layouts[layout];
+  for (int32_t id : L->eset_ids) {
+    E.esets_host[id] = EsetDev{nullptr, nullptr};
+    E.eset_layout[id] = -1;
+  }
+  delete L;
+  E.layouts[layout] = nullptr;
+  E.layouts_dirty = E.esets_dirty = true;
   return KMC_OK;
 }
 
@@ -355,8 +398,18 @@ int batch_create_locked(const kmc_traj* t, int64_t ntraj, int32_t* out) {
     delete B;
     return fail(KMC_ERR_DEVICE, "hipEventCreate failed");
   }
-  E.batches.push_back(B);
-  *out = (int32_t)(E.batches.size() - 1);
+  int32_t id = -1;
+  for (size_t i = 0; i < E.batches.size(); ++i)
+    if (!E.batches[i]) {
+      id = (int32_t)i;
+      break;
+    }
+  if (id < 0) {
+    id = (int32_t)E.batches.size();
+    E.batches.push_back(nullptr);
+  }
+  E.batches[id] = B;
+  *out = id;
   return KMC_OK;
 }
 
@@ -620,8 +673,6 @@ int kmc_reset(void) {
   E.batches.clear();
   for (Layout* l : E.layouts) delete l;
   E.layouts.clear();
-  for (DevBuf* b : E.eset_bufs) delete b;
-  E.eset_bufs.clear();
   E.esets_host.clear();
   E.eset_layout.clear();
   E.layouts_dirty = E.esets_dirty = true;
@@ -637,6 +688,12 @@ int kmc_layout_create(int64_t N, int64_t P, double nu, double kT, double I_0, do
   if (st) return st;
   return layout_create_locked(N, P, nu, kT, I_0, R, distances, transitions_constant,
                               prune_threshold, layout_out);
+}
+
+int kmc_layout_destroy(int32_t layout) {
+  Engine& E = eng();
+  std::lock_guard<std::mutex> lk(E.mu);
+  return layout_destroy_locked(layout);
 }
 
 int kmc_eset_create(int32_t layout, int64_t count, const double* E_constant, const double* V,
@@ -765,18 +822,9 @@ int kmc_simulate_discrete(int64_t N, int64_t P, double nu, double kT, double I_0
     (void)hipStreamSynchronize(E.stream);
     if (bid >= 0 && bid < (int)E.batches.size()) {
       delete E.batches[bid];
-      E.batches.pop_back();
+      E.batches[bid] = nullptr;
     }
-    if (!E.eset_bufs.empty() && eid >= 0) {
-      delete E.eset_bufs.back();
-      E.eset_bufs.pop_back();
-      E.esets_host.pop_back();
-      E.eset_layout.pop_back();
-      E.esets_dirty = true;
-    }
-    delete E.layouts.back();
-    E.layouts.pop_back();
-    E.layouts_dirty = true;
+    layout_destroy_locked(lid);
     return code;
   };
   eid = -1;

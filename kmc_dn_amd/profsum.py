@@ -18,6 +18,11 @@ def kernel_stats(con) -> list[tuple]:
     return rows
 
 
+def dispatches(con) -> list[tuple]:
+    return con.execute("select name, grid_x, workgroup_x, lds_size, vgpr_count, "
+                       "scratch_size, duration from kernels order by start").fetchall()
+
+
 def counter_stats(con) -> dict:
     acc = defaultdict(list)
     try:
@@ -32,6 +37,8 @@ def counter_stats(con) -> dict:
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("db", nargs="+")
+    ap.add_argument("--dispatches", action="store_true", help="list every dispatch")
+    ap.add_argument("--match", default="", help="only kernels whose name contains this")
     args = ap.parse_args(argv)
     for path in args.db:
         con = sqlite3.connect(path)
@@ -43,6 +50,12 @@ def main(argv=None) -> int:
             print("|---|---|---|---|---|")
             for name, calls, tot, avg, pct in ks:
                 print(f"| `{name}` | {calls} | {tot / 1e3:.3f} | {avg / 1e3:.3f} | {pct:.2f} |")
+        if args.dispatches:
+            print("| kernel | grid | workgroup | LDS B | VGPRs | scratch | ms |")
+            print("|---|---|---|---|---|---|---|")
+            for name, gx, wx, lds, vg, scr, dur in dispatches(con):
+                if args.match in name:
+                    print(f"| `{name}` | {gx} | {wx} | {lds} | {vg} | {scr} | {dur / 1e6:.3f} |")
         cs = counter_stats(con)
         if cs:
             print("| kernel | counter | dispatches | average per dispatch |")

@@ -1,0 +1,90 @@
+"""The exact-name cgo exports, called the way the reference's ctypes code calls them.
+
+goSimulation/pythonBind.py:64-90 declares 7 doubles (nu, kT, I_0, R, time -- one
+more than Go takes), passes hops as a 32-bit c_int and the slices as GoSlice
+structs by value; goSimulation/parrallelSimulationBind.py:50-79 passes 14 slices.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from kmc_dn_amd import _lib
+from tests.netgen import random_network
+
+pytestmark = pytest.mark.gpu
+
+
+class GoSlice(C.Structure):
+    _fields_ = [("data", C.POINTER(C.c_double)), ("len", C.c_longlong), ("cap", C.c_longlong)]
+
+
+def go_slice(arr):
+    a = (C.c_double * len(arr))(*[float(x) for x in arr])
+    return GoSlice(a, len(arr), len(arr)), a
+
+
+def _read(s, n):
+    return np.array([s.data[i] for i in range(n)])
+
+
+def test_wrapper_simulate_record_like_pythonbind(gpu_engine):
+    net = random_network(4, 20, 6, I_0=20.0, a=0.3, vmax=2.0)
+    N, P = 20, 6
+    S = N + P
+    lib = C.CDLL(_lib.LIB_PATH)
+    f = lib.wrapperSimulateRecord
+    f.argtypes = [C.c_longlong, C.c_longlong, C.c_double, C.c_double, C.c_double, C.c_double,
+                  C.c_double, GoSlice, GoSlice, GoSlice, GoSlice, GoSlice, GoSlice, C.c_int,
+                  C.c_bool, GoSlice, GoSlice]
+    f.restype = C.c_double
+    keep = []
+    def gs(x):
+        s, a = go_slice(np.ravel(x))
+        keep.append(a)
+        return s
+    occ = gs(net["occupation"].astype(float))
+    eo = gs(np.zeros(P))
+    traffic = gs(np.zeros(S * S))
+    avg = gs(np.zeros(N))
+    t = f(N, P, 1.0, 1.0, net["I_0"], net["R"], 0.0, occ, gs(net["distances"]),
+          gs(net["E_constant"]), gs(net["transitions_constant"]), eo, gs(net["site_energies"]),
+          20000, True, traffic, avg)
+    assert np.isfinite(t) and t > 0, _lib.load().kmc_last_error()
+    e = _read(eo, P)
+    tr = _read(traffic, S * S).reshape(S, S)
+    assert np.array_equal(tr, -tr.T)
+    assert np.array_equal(tr[:, N:].sum(0), e)          # net hops into each electrode
+    a = _read(avg, N) / t
+    assert np.all((a >= 0) & (a <= 1 + 1e-9))
+    # the occupation slice is input only, as in the Go code
+    assert np.array_equal(_read(occ, N), net["occupation"].astype(float))
+
+
+def test_parallel_simulations_like_parrallelsimulationbind(gpu_engine):
+    nets = [random_network(s, n, p, I_0=20.0, a=0.3, vmax=2.0) for s, n, p in
+            [(1, 12, 4), (2, 25, 8), (3, 40, 6)]]
+    cols = {k: [] for k in ["N", "P", "nu", "kT", "I_0", "R", "occ", "d", "E", "tc", "eo", "hops",
+                            "time", "se"]}
+    for n in nets:
+        cols["N"].append(n["N"]); cols["P"].append(n["P"]); cols["nu"].append(1.0)
+        cols["kT"].append(1.0); cols["I_0"].append(n["I_0"]); cols["R"].append(n["R"])
+        cols["occ"].extend(n["occupation"].astype(float)); cols["d"].extend(n["distances"].ravel())
+        cols["E"].extend(n["E_constant"]); cols["tc"].extend(n["transitions_constant"].ravel())
+        cols["eo"].extend(np.zeros(n["P"])); cols["hops"].append(10000.0); cols["time"].append(0.0)
+        cols["se"].extend(n["site_energies"])
+    keep, slices = [], {}
+    for k, v in cols.items():
+        s, a = go_slice(v)
+        keep.append(a)
+        slices[k] = s
+    lib = C.CDLL(_lib.LIB_PATH)
+    lib.parallelSimulations.argtypes = [GoSlice] * 14
+    lib.parallelSimulations.restype = C.c_longlong
+    order = ["N", "P", "nu", "kT", "I_0", "R", "occ", "d", "E", "tc", "eo", "hops", "time", "se"]
+    rc = lib.parallelSimulations(*[slices[k] for k in order])
+    assert rc == 0, _lib.load().kmc_last_error()
+    times = _read(slices["time"], 3)
+    assert np.all(times > 0)
+    eo = _read(slices["eo"], sum(cols["P"]))
+    assert np.all(np.isfinite(eo)) and np.abs(eo).sum() > 0

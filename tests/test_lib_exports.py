@@ -12,6 +12,7 @@ def declared_functions():
     src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
     names = re.findall(r"^\s*(?:const\s+)?(?:int|double|void|int64_t|char)\s*\*?\s*(\w+)\s*\(",
                        src, flags=re.M)
+    assert "parallelSimulations" in names and "wrapperSimulateRecord" in names
     return sorted(set(names))
 
 
