@@ -97,7 +97,7 @@ def iv_leg(lay, args) -> dict:
     all points x replicas in one launch on the shared layout."""
     from kmc_dn_amd import _lib, iv, workloads
     sw = workloads.iv_sweep(0, args.iv_points, args.iv_replicas, args.iv_hops)
-    b = iv.iv_batch(lay, sw.E_sets, sw.V_sets, sw.replicas, sw.hops, sw.net.occupation)
+    b, _ = iv.iv_batch(lay, sw.E_sets, sw.V_sets, sw.replicas, sw.hops, sw.net.occupation)
     t0 = time.perf_counter()
     b.run(precision=_lib.FP32_BATCHED)
     ms = b.sync()

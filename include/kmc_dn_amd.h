@@ -98,6 +98,9 @@ int kmc_layout_destroy(int32_t layout);
 int kmc_eset_create(int32_t layout, int64_t count, const double* E_constant, const double* V,
                     int32_t* first_out);
 
+/* Frees one energy set (its memory is released with the last set of its upload). */
+int kmc_eset_destroy(int32_t eset);
+
 typedef struct kmc_traj {
   int32_t layout;             /* from kmc_layout_create */
   int32_t eset;               /* from kmc_eset_create (same layout) */

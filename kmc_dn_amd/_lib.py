@@ -68,6 +68,7 @@ SIGNATURES = {
                                     _dp, _dp, C.c_double, _i32p]),
     "kmc_layout_destroy": (C.c_int, [_i32]),
     "kmc_eset_create": (C.c_int, [_i32, _i64, _dp, _dp, _i32p]),
+    "kmc_eset_destroy": (C.c_int, [_i32]),
     "kmc_batch_create": (C.c_int, [C.POINTER(kmc_traj), _i64, _i32p]),
     "kmc_batch_run": (C.c_int, [_i32, C.POINTER(kmc_run_opts)]),
     "kmc_batch_sync": (C.c_int, [_i32, _dp]),

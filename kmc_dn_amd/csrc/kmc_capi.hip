@@ -75,8 +75,10 @@ struct DevBuf {
 
 struct Layout {
   int N = 0, P = 0, S = 0;
-  std::vector<DevBuf*> eset_bufs;   // energy sets created on this layout
-  std::vector<int32_t> eset_ids;
+  std::vector<DevBuf*> eset_bufs;   // energy-set buffers created on this layout
+  std::vector<int> eset_live;       // live energy sets per buffer
+  std::vector<int32_t> eset_ids;    // energy-set id -> (position here)
+  std::vector<int> eset_buf;        // buffer index of each entry of eset_ids
   ~Layout() {
     for (DevBuf* b : eset_bufs) delete b;
   }
@@ -309,10 +311,12 @@ int eset_create_locked(int32_t layout, int64_t count, const double* Ec, const do
     e.E = b->as<double>() + (size_t)i * L->N;
     e.V = b->as<double>() + nE + (size_t)i * L->P;
     L->eset_ids.push_back(base + (int32_t)i);
+    L->eset_buf.push_back((int)L->eset_bufs.size());
     E.esets_host[base + i] = e;
     E.eset_layout[base + i] = layout;
   }
   L->eset_bufs.push_back(b);
+  L->eset_live.push_back((int)count);
   E.esets_dirty = true;
   return KMC_OK;
 }
@@ -330,6 +334,29 @@ int layout_destroy_locked(int32_t layout) {
   delete L;
   E.layouts[layout] = nullptr;
   E.layouts_dirty = E.esets_dirty = true;
+  return KMC_OK;
+}
+
+int eset_destroy_locked(int32_t id) {
+  Engine& E = eng();
+  if (id < 0 || id >= (int)E.eset_layout.size() || E.eset_layout[id] < 0)
+    return fail(KMC_ERR_ARGS, "bad energy-set id %d", id);
+  Layout* L = E.layouts[E.eset_layout[id]];
+  for (size_t k = 0; k < L->eset_ids.size(); ++k) {
+    if (L->eset_ids[k] != id) continue;
+    const int bi = L->eset_buf[k];
+    L->eset_ids.erase(L->eset_ids.begin() + k);
+    L->eset_buf.erase(L->eset_buf.begin() + k);
+    if (--L->eset_live[bi] == 0) {
+      (void)hipStreamSynchronize(E.stream);
+      delete L->eset_bufs[bi];
+      L->eset_bufs[bi] = nullptr;
+    }
+    break;
+  }
+  E.esets_host[id] = EsetDev{nullptr, nullptr};
+  E.eset_layout[id] = -1;
+  E.esets_dirty = true;
   return KMC_OK;
 }
 
@@ -694,6 +721,12 @@ int kmc_layout_destroy(int32_t layout) {
   Engine& E = eng();
   std::lock_guard<std::mutex> lk(E.mu);
   return layout_destroy_locked(layout);
+}
+
+int kmc_eset_destroy(int32_t eset) {
+  Engine& E = eng();
+  std::lock_guard<std::mutex> lk(E.mu);
+  return eset_destroy_locked(eset);
 }
 
 int kmc_eset_create(int32_t layout, int64_t count, const double* E_constant, const double* V,

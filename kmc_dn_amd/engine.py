@@ -84,6 +84,17 @@ class Layout:
                                          C.byref(first)))
         return np.arange(first.value, first.value + E.shape[0], dtype=np.int32)
 
+    @staticmethod
+    def drop_energy_sets(ids) -> None:
+        for i in np.atleast_1d(ids):
+            L.check(L.load().kmc_eset_destroy(int(i)))
+
+    def close(self) -> None:
+        """Free the device tables and every energy set of this layout."""
+        if self.id >= 0:
+            L.check(L.load().kmc_layout_destroy(self.id))
+            self.id = -1
+
 
 @dataclass
 class BatchResult:

@@ -21,7 +21,8 @@ def iv_batch(layout: engine.Layout, E_sets: np.ndarray, V_sets: np.ndarray, repl
              hops: int, occupation, prehops: int = 0, seed0: int = 0):
     """Upload the energy sets and build the batch (voltage-major, replicas inner).
 
-    Trajectory k = i * replicas + r has Philox key seed0 + k.
+    Trajectory k = i * replicas + r has Philox key seed0 + k.  Returns the batch and
+    the energy-set ids (free them with engine.Layout.drop_energy_sets when done).
     """
     ids = layout.energy_sets(E_sets, V_sets)
     nV = len(ids)
@@ -29,7 +30,7 @@ def iv_batch(layout: engine.Layout, E_sets: np.ndarray, V_sets: np.ndarray, repl
     t = engine.make_trajs(K, layout.id, np.repeat(ids, replicas),
                           np.arange(seed0, seed0 + K, dtype=np.uint64), hops, occupation,
                           prehops=prehops)
-    return engine.Batch(t)
+    return engine.Batch(t), ids
 
 
 def currents_from(res: engine.BatchResult, nV: int, replicas: int, P: int):
@@ -57,11 +58,12 @@ def IV(kmc, electrode: int, voltagelist, hops: int = 0, prehops: int = 0, replic
     E = basis.eV(Vs, kmc.mu) + kmc.comp_constant[None, :]
     lay = kmc._layout()
     seed = int(np.random.randint(0, 2 ** 31)) if seed0 is None else seed0
-    b = iv_batch(lay, E, Vs, replicas, int(hops), kmc.occupation, int(prehops), seed)
+    b, ids = iv_batch(lay, E, Vs, replicas, int(hops), kmc.occupation, int(prehops), seed)
     b.run(precision=precision)
     b.sync()
     r = b.results()
     b.close()
+    engine.Layout.drop_energy_sets(ids)
     for st in np.unique(r.status):
         _lib.check(int(st))
     mean, se = currents_from(r, len(volts), replicas, kmc.P)

@@ -288,9 +288,10 @@ class kmc_dn:
                 self.traffic = traffic
                 self.average_occupation = [x / self.time for x in oit]
         else:
-            lay = self._layout() if prune_threshold == 0 else engine.Layout(
-                self.N, self.P, self.nu, self.kT, self.I_0, self.R, self.distances,
-                self.transitions_constant, prune_threshold)
+            pruned = prune_threshold != 0
+            lay = engine.Layout(self.N, self.P, self.nu, self.kT, self.I_0, self.R,
+                                self.distances, self.transitions_constant,
+                                prune_threshold) if pruned else self._layout()
             es = lay.energy_sets(self.E_constant, self.site_energies[self.N:])
             # the Go wrappers start from an empty network (simulationWrapper.go:134-141)
             b = engine.Batch(engine.make_trajs(1, lay.id, es[0], self._draw_seed(), hops,
@@ -302,6 +303,10 @@ class kmc_dn:
             if record:
                 self.traffic = b.traffic(self.N + self.P)[0]
             b.close()
+            if pruned:
+                lay.close()
+            else:
+                engine.Layout.drop_energy_sets(es)
             _lib.check(int(r.status[0]))
             self.time = float(r.time[0])
             self.electrode_occupation = r.eo[0, :self.P].astype(np.int64)
