@@ -90,6 +90,12 @@ int kmc_layout_create(int64_t N, int64_t P, double nu, double kT, double I_0, do
                       const double* distances, const double* transitions_constant,
                       double prune_threshold, int32_t* layout_out);
 
+/* Attaches site coordinates (S*3, acceptors then electrodes) and the localisation
+ * length ab to a layout, enabling the positions form (table_form 3).  The layout's
+ * distances and transitions_constant must equal the reference closed forms
+ * (kmc_dopant_networks.py:657-695, :824-830) within 1e-9 relative. */
+int kmc_layout_set_positions(int32_t layout, double ab, const double* positions);
+
 /* Frees a layout's tables and every energy set created on it (ids may be reused). */
 int kmc_layout_destroy(int32_t layout);
 
@@ -120,12 +126,26 @@ typedef struct kmc_run_opts {
   int32_t post_hop_record;    /* 1: numba convention (post-hop state), 0: pre-hop */
   int32_t grid_limit;         /* FP32: max resident workgroups; 0 = auto */
   int32_t use_graph;          /* 1: launch through a captured hipGraph */
-  int32_t table_form;         /* FP32 LDS pair table: 0 auto, 1 {lt, lt+c} + c table
-                                 (fewer VALU ops), 2 {lt, c} (smaller LDS) */
+  int32_t table_form;         /* FP32 pair terms: 0 auto, 1 {lt, lt+c} + c table in LDS
+                                 (fewer VALU ops), 2 {lt, c} table (smaller LDS),
+                                 3 computed from site positions (no table; needs
+                                 kmc_layout_set_positions on every layout) */
+  int32_t resume;             /* FP32: run each trajectory's [h_begin, h_end) set by
+                                 kmc_batch_set_state, continuing from that state */
+  int32_t pad;
 } kmc_run_opts;
 
 /* Uploads trajectory descriptors and allocates device outputs. */
 int kmc_batch_create(const kmc_traj* trajs, int64_t ntraj, int32_t* batch_out);
+/* Replaces the descriptors of a batch (ntraj up to its creation size; buffers
+ * are reused, no allocation when it fits) -- for schedulers that refill slots. */
+int kmc_batch_update(int32_t batch, const kmc_traj* trajs, int64_t ntraj);
+/* Resume state per trajectory (caller order; any pointer may be NULL = keep):
+ * absolute hop range [h_begin, h_end) of the next resume run, and the state it
+ * continues from: time, eo[ntraj*KMC_PMAX], occ[ntraj*KMC_OCCW].  A range that
+ * starts at a multiple of the recompute interval continues bit-exactly. */
+int kmc_batch_set_state(int32_t batch, const int64_t* h_begin, const int64_t* h_end,
+                        const double* time, const int64_t* eo, const uint64_t* occ);
 /* Enqueues the batch on the engine stream (asynchronous). */
 int kmc_batch_run(int32_t batch, const kmc_run_opts* opts);
 /* Waits for the last run; elapsed_ms (optional) = HIP-event time of its kernels. */

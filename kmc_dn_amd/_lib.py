@@ -38,7 +38,8 @@ class kmc_run_opts(C.Structure):
     _fields_ = [("precision", C.c_int32), ("waves_per_group", C.c_int32),
                 ("recompute_interval", C.c_int32), ("record", C.c_int32),
                 ("post_hop_record", C.c_int32), ("grid_limit", C.c_int32),
-                ("use_graph", C.c_int32), ("table_form", C.c_int32)]
+                ("use_graph", C.c_int32), ("table_form", C.c_int32),
+                ("resume", C.c_int32), ("pad", C.c_int32)]
 
 
 class GoSlice(C.Structure):
@@ -67,6 +68,9 @@ SIGNATURES = {
     "kmc_layout_create": (C.c_int, [_i64, _i64, C.c_double, C.c_double, C.c_double, C.c_double,
                                     _dp, _dp, C.c_double, _i32p]),
     "kmc_layout_destroy": (C.c_int, [_i32]),
+    "kmc_layout_set_positions": (C.c_int, [_i32, C.c_double, _dp]),
+    "kmc_batch_update": (C.c_int, [_i32, C.POINTER(kmc_traj), _i64]),
+    "kmc_batch_set_state": (C.c_int, [_i32, _i64p, _i64p, _dp, _i64p, _u64p]),
     "kmc_eset_create": (C.c_int, [_i32, _i64, _dp, _dp, _i32p]),
     "kmc_eset_destroy": (C.c_int, [_i32]),
     "kmc_batch_create": (C.c_int, [C.POINTER(kmc_traj), _i64, _i32p]),

@@ -5,14 +5,16 @@
 //
 //  * Each lane owns sites s = lane + 64*c (c < C).  Site energies live in
 //    registers as e~ = se * log2(e)/kT; electrodes hold V * log2(e)/kT.
-//  * The pair table of the group's layout sits in LDS, shared by the W waves of
-//    the workgroup, which run W trajectories of that layout.  With
-//    lt = log2(nu * tc) and c = scaled Coulomb term I_0*R/d, a rate is
+//  * With lt = log2(nu * tc) and c = scaled Coulomb term I_0*R/d a rate is
 //        2^(lt + min(0, e~_src - e~_dst + c)) = 2^min(lt, b + e~_src - e~_dst),
 //    b = lt + c: one v_exp_f32 per pair and no FP32 underflow of the 10^-50
-//    prefactors.  Table form BT: {lt, b} pairs plus a c table for the energy
-//    update (fewest VALU ops); form !BT: {lt, c} pairs only (smallest LDS,
-//    for the largest networks).
+//    prefactors.  Where the pair terms come from is a policy:
+//      TabB  {lt, b} pair table + c table in LDS, shared by the W waves of a
+//            workgroup that run W trajectories of one layout (fewest VALU ops);
+//      TabC  {lt, c} pair table in LDS (smaller; largest networks);
+//      Pos   positions only (16 B per site in LDS per wave): d, lt and c are
+//            computed per pair, ~2.5x the VALU work but no table, so waves of
+//            different layouts fill a CU independently.
 //  * Per hop, the smaller of {sources} (occupied acceptors + electrodes) and
 //    {targets} (empty acceptors + electrodes) is walked four members at a time
 //    with a scalar loop while the lanes cover every site of the other role:
@@ -23,14 +25,18 @@
 //  * Hop: occupation bits flip in SGPRs; energies get the O(N) rank-1 Coulomb
 //    update of makeJump (simulation.go:107-130); electrode tallies stay in the
 //    electrode lanes' registers; the clock is accumulated in FP64.
-//  * Every `recompute_interval` hops the energies are rebuilt in FP64 from
-//    E_constant and the I_0*R/d table, cancelling FP32 drift of the updates
-//    (the Go reference never re-synchronises).
+//  * At every multiple of `recompute_interval` hops the energies are rebuilt in
+//    FP64 from E_constant and the I_0*R/d table, cancelling FP32 drift of the
+//    updates (the Go reference never re-synchronises).
 //  * Uniforms come from one Philox block per lane every 64 hops (lane k holds
 //    hop h0+k), so the RNG costs two readlanes per hop.
+//  * A trajectory runs the absolute hop range [h_begin, h_end) of its budget
+//    and, with `resume`, starts from the state its previous launch wrote.
+//    Ranges that start at multiples of the recompute interval continue the
+//    trajectory exactly as one long launch would (same energies, same stream).
 //
-// The workgroup pulls groups (layout, first trajectory, count) from a global
-// atomic queue until it is drained, so one launch serves uneven batches.
+// Workgroups (table policies) or waves (Pos) pull work from a global atomic
+// queue until it is drained, so one launch serves uneven batches.
 #include "kmc_device.h"
 
 namespace kmc {
@@ -57,50 +63,126 @@ __device__ __forceinline__ void recompute_energies(float (&se)[C], const uint64_
   }
 }
 
-// log2 of a pair rate from its table entry and the source-minus-target energy.
-template <bool BT>
-__device__ __forceinline__ float log2_rate(float2 tv, float diff) {
-  if constexpr (BT) return fminf(tv.x, tv.y + diff);
-  else return tv.x + fminf(0.f, diff + tv.y);
-}
+// ---------------------------------------------------------------------------
+// Pair-term policies.  For a uniform member u of the walked set (u < 0: none)
+// and the lane sites s = lane + 64c, pair(pre(u), c, diff) is log2 of the rate
+// of the pair (src, dst) with e~_src - e~_dst = diff; coul(u, c) is c(s, u) for
+// the rank-1 energy update (u an acceptor).  prefetch(u) issues the loads that
+// pair() needs, so four members can be in flight together.
+// ---------------------------------------------------------------------------
+template <int C, bool BT>
+struct TabPolicy {
+  const float2* tab;   // [S*S + 192]: entry (s, u) at u*S + s; column S is all -inf
+  const float* ctab;   // BT: [N*S + 192] c at u*S + s
+  int S, lane;
+  struct Pre {
+    float2 e[C];
+  };
+  __device__ __forceinline__ Pre prefetch(int u) const {
+    Pre p;
+    const float2* col = tab + (u < 0 ? S * S : u * S) + lane;
+#pragma unroll
+    for (int c = 0; c < C; ++c) p.e[c] = col[64 * c];
+    return p;
+  }
+  __device__ __forceinline__ float pair(const Pre& p, int c, float diff) const {
+    if constexpr (BT) return fminf(p.e[c].x, p.e[c].y + diff);
+    else return p.e[c].x + fminf(0.f, diff + p.e[c].y);
+  }
+  __device__ __forceinline__ float coul(int u, int c) const {
+    if constexpr (BT) return ctab[u * S + lane + 64 * c];
+    else return tab[u * S + lane + 64 * c].y;
+  }
+};
+
+template <int C>
+struct PosPolicy {
+  const float4* pos;   // [S] per wave: x, y, z, 1 for acceptors / 0 for electrodes
+  float px[C], py[C], pz[C], kcl[C];   // the lane sites' coordinates and c scale
+  bool lane_el[C];
+  float l0, mkl;       // lt = l0 + mkl * d
+  int lane;
+  struct Pre {
+    float4 q;
+  };
+  __device__ __forceinline__ void init(const float4* p, const LayoutDev& L, int ln) {
+    pos = p;
+    lane = ln;
+    l0 = L.pos_l0;
+    mkl = L.pos_mkl;
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      const int s = lane + 64 * c;
+      const float4 q = s < L.S ? pos[s] : make_float4(1e6f, 1e6f, 1e6f, 0.f);
+      px[c] = q.x;
+      py[c] = q.y;
+      pz[c] = q.z;
+      kcl[c] = q.w * L.pos_kc;
+      lane_el[c] = s >= L.N && s < L.S;
+    }
+  }
+  __device__ __forceinline__ Pre prefetch(int u) const {
+    Pre p;
+    p.q = u < 0 ? make_float4(0.f, 0.f, 0.f, -1.f) : pos[u];
+    return p;
+  }
+  __device__ __forceinline__ float dist_rinv(const float4& q, int c, float& rinv) const {
+    const float dx = px[c] - q.x, dy = py[c] - q.y, dz = pz[c] - q.z;
+    const float d2 = fmaxf(dx * dx + dy * dy + dz * dz, 1e-30f);
+    rinv = __builtin_amdgcn_rsqf(d2);
+    return d2 * rinv;
+  }
+  __device__ __forceinline__ float pair(const Pre& p, int c, float diff) const {
+    float rinv;
+    const float d = dist_rinv(p.q, c, rinv);
+    const float lt = fmaf(mkl, d, l0);
+    const float cc = rinv * kcl[c] * p.q.w;
+    const float x = lt + fminf(0.f, diff + cc);
+    // padding members and electrode-electrode pairs (an electrode with itself
+    // included) have no rate
+    return (p.q.w < 0.f || (p.q.w == 0.f && lane_el[c])) ? -INFINITY : x;
+  }
+  __device__ __forceinline__ float coul(int u, int c) const {
+    const float4 q = pos[u];
+    float rinv;
+    (void)dist_rinv(q, c, rinv);
+    return (u == lane + 64 * c) ? 0.f : rinv * kcl[c] * q.w;
+  }
+};
 
 // Row sums over the uniform set U (bits of umask), lanes covering the other role.
 // UDST: U = targets, lanes = sources; otherwise U = sources, lanes = targets.
-// Four members of U per step so their LDS reads are in flight together; a
-// missing member reads the all -inf column past the table (index S) and adds 0.
-template <int C, bool UDST, bool BT>
-__device__ __forceinline__ void row_sums(const float2* __restrict__ tab, int S,
-                                         const uint64_t (&umask)[C], const float (&v)[C],
-                                         const float (&se)[C], int lane, float (&R)[C]) {
+template <int C, bool UDST, class Pol>
+__device__ __forceinline__ void row_sums(const Pol& pol, const uint64_t (&umask)[C],
+                                         const float (&v)[C], const float (&se)[C],
+                                         float (&R)[C]) {
   constexpr int U = 4;
 #pragma unroll
   for (int cu = 0; cu < C; ++cu) {
     uint64_t m = umask[cu];
     while (m) {
-      int b[U], col[U];
+      int b[U], uu[U];
 #pragma unroll
       for (int k = 0; k < U; ++k) {
         if (m) {
           b[k] = __builtin_ctzll(m);
           m &= m - 1;
-          col[k] = (64 * cu + b[k]) * S;
+          uu[k] = 64 * cu + b[k];
         } else {
           b[k] = 0;
-          col[k] = S * S;
+          uu[k] = -1;
         }
       }
-      float2 tv[U][C];
+      typename Pol::Pre pre[U];
 #pragma unroll
-      for (int k = 0; k < U; ++k)
-#pragma unroll
-        for (int c = 0; c < C; ++c) tv[k][c] = tab[col[k] + lane + 64 * c];
+      for (int k = 0; k < U; ++k) pre[k] = pol.prefetch(uu[k]);
 #pragma unroll
       for (int k = 0; k < U; ++k) {
         const float su = readlane_f32(se[cu], b[k]);
 #pragma unroll
         for (int c = 0; c < C; ++c) {
           const float diff = UDST ? (v[c] - su) : (su - v[c]);
-          R[c] += fast_exp2(log2_rate<BT>(tv[k][c], diff));
+          R[c] += fast_exp2(pol.pair(pre[k], c, diff));
         }
       }
     }
@@ -117,39 +199,42 @@ __device__ __forceinline__ float pick_lane_value(const float (&x)[C], int site) 
   return r;
 }
 
-template <int C, bool REC, bool BT>
-__device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L,
-                               const float2* __restrict__ tab, const float* __restrict__ ctab,
+template <int C, bool REC, class Pol>
+__device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, const Pol& pol,
                                int lane) {
   const TrajDev td = p.trajs[t];
   const EsetDev es = p.esets[td.eset];
   const int N = L.N, S = L.S;
-  uint64_t occ[OCCW];
-#pragma unroll
-  for (int w = 0; w < OCCW; ++w) occ[w] = td.occ[w];
+  TrajOut* o = p.out + t;
+  const int64_t total_hops = td.prehops + td.hops;
+  const int64_t h_begin = p.resume ? td.h_begin : 0;
+  const int64_t h_end = p.resume ? (td.h_end < total_hops ? td.h_end : total_hops) : total_hops;
+  const bool cont = p.resume && h_begin > 0;
 
+  uint64_t occ[OCCW];
   float se[C];
   int eo[C];
   double ot[C];
 #pragma unroll
+  for (int w = 0; w < OCCW; ++w) occ[w] = cont ? o->occ[w] : td.occ[w];
+#pragma unroll
   for (int c = 0; c < C; ++c) {
-    eo[c] = 0;
-    ot[c] = 0.0;
+    const int s = lane + 64 * c;
+    eo[c] = (cont && s >= N && s < S) ? (int)o->eo[s - N] : 0;
+    ot[c] = (cont && REC && p.occ_time && s < N) ? p.occ_time[(size_t)t * NMAX + s] : 0.0;
   }
+  double time = cont ? o->time : 0.0;
+  int32_t status = cont ? o->status : (int32_t)ST_OK;
   recompute_energies<C>(se, occ, L, es, lane);
 
-  double time = 0.0;
-  int32_t status = ST_OK;
-  const int64_t total_hops = td.prehops + td.hops;
   const int K = p.recompute_interval;
-  int since = 0;
   float lg_lane = 0.f;    // log2(1 - u_clock) of hop (h & ~63) + lane
   double ue_lane = 0.0;   // event uniform of that hop
-  int64_t h = 0;
-  for (; h < total_hops; ++h) {
-    if ((h & 63) == 0) {
+  int64_t h = h_begin;
+  for (; h < h_end && status == ST_OK; ++h) {
+    if (h == h_begin || (h & 63) == 0) {
       double uc;
-      stream_uniforms(td.seed, td.hop0 + (uint64_t)(h + lane), uc, ue_lane);
+      stream_uniforms(td.seed, td.hop0 + (uint64_t)((h & ~(int64_t)63) + lane), uc, ue_lane);
       lg_lane = fast_log2((float)(1.0 - uc));
     }
     if (h == td.prehops) {
@@ -160,11 +245,7 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L,
         ot[c] = 0.0;
       }
     }
-    if (since == K) {
-      recompute_energies<C>(se, occ, L, es, lane);
-      since = 0;
-    }
-    ++since;
+    if (h != h_begin && h % K == 0) recompute_energies<C>(se, occ, L, es, lane);
 
     // roles of the lanes' sites
     uint64_t srcm[C], dstm[C];
@@ -189,8 +270,8 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L,
       const bool lane_ok = udst ? ((srcm[c] >> lane) & 1ull) : ((dstm[c] >> lane) & 1ull);
       v[c] = lane_ok ? se[c] : (udst ? -INFINITY : INFINITY);
     }
-    if (udst) row_sums<C, true, BT>(tab, S, dstm, v, se, lane, R);
-    else row_sums<C, false, BT>(tab, S, srcm, v, se, lane, R);
+    if (udst) row_sums<C, true>(pol, dstm, v, se, R);
+    else row_sums<C, false>(pol, srcm, v, se, R);
 
     // FP64 prefix over lane rows
     double incl[C];
@@ -236,14 +317,14 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L,
 
     // partner of `sel`: lane-parallel pass over its row, in site order
     const float s_sel = pick_lane_value<C>(se, sel);
-    const float2* row = tab + sel * S + lane;
+    const typename Pol::Pre pre = pol.prefetch(sel);
     float run = 0.f;
     int pick = -1, last_nz = -1;
 #pragma unroll
     for (int c = 0; c < C; ++c) {
       const bool inU = udst ? ((dstm[c] >> lane) & 1ull) : ((srcm[c] >> lane) & 1ull);
       const float diff = udst ? (s_sel - se[c]) : (se[c] - s_sel);
-      float r = fast_exp2(log2_rate<BT>(row[64 * c], diff));
+      float r = fast_exp2(pol.pair(pre, c, diff));
       r = inU ? r : 0.f;
       const float cum = wave_scan_f32(r) + run;
       const uint64_t nz = ballot(r > 0.f);
@@ -277,8 +358,7 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L,
     if (src < N) {
       occ[src >> 6] &= ~(1ull << (src & 63));
 #pragma unroll
-      for (int c = 0; c < C; ++c)
-        se[c] -= BT ? ctab[src * S + lane + 64 * c] : tab[src * S + lane + 64 * c].y;
+      for (int c = 0; c < C; ++c) se[c] -= pol.coul(src, c);
     } else {
 #pragma unroll
       for (int c = 0; c < C; ++c)
@@ -287,8 +367,7 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L,
     if (dst < N) {
       occ[dst >> 6] |= 1ull << (dst & 63);
 #pragma unroll
-      for (int c = 0; c < C; ++c)
-        se[c] += BT ? ctab[dst * S + lane + 64 * c] : tab[dst * S + lane + 64 * c].y;
+      for (int c = 0; c < C; ++c) se[c] += pol.coul(dst, c);
     } else {
 #pragma unroll
       for (int c = 0; c < C; ++c)
@@ -304,7 +383,6 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L,
     time += dt;
   }
 
-  TrajOut* o = p.out + t;
   if (lane == 0) {
     o->time = time;
     o->hops_done = h > td.prehops ? h - td.prehops : 0;
@@ -320,8 +398,9 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L,
   }
 }
 
-// LDS: the group layout's pair table (S*S + 192 float2) and, for BT, its c
-// table (N*S + 192 floats).
+// Table policies: the workgroup's W waves run W trajectories of one layout and
+// share its table in LDS: the pair table (S*S + 192 float2) and, for BT, the
+// c table (N*S + 192 floats).
 template <int C, int W, bool REC, bool BT>
 __global__ void __launch_bounds__(64 * W) kmc_batch_fp32_kernel(BatchParams p) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -351,15 +430,43 @@ __global__ void __launch_bounds__(64 * W) kmc_batch_fp32_kernel(BatchParams p) {
       if (threadIdx.x == 0) s_layout = gd.layout;
     }
     __syncthreads();
-    if (wave < gd.count) run_trajectory<C, REC, BT>(p, gd.first + wave, L, tab, ctab, lane);
+    if (wave < gd.count) {
+      const TabPolicy<C, BT> pol{tab, ctab, L.S, lane};
+      run_trajectory<C, REC>(p, gd.first + wave, L, pol, lane);
+    }
   }
 }
 
-#define KMC_INST(C, W)                                                      \
+// Positions policy: every wave pulls its own groups and keeps only its layout's
+// site coordinates in LDS (at most 64*C float4), so a workgroup mixes layouts.
+template <int C, int W, bool REC>
+__global__ void __launch_bounds__(64 * W) kmc_batch_pos_kernel(BatchParams p) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  float4* pos = reinterpret_cast<float4*>(smem) + wave * (64 * C);
+  for (;;) {
+    int g = 0;
+    if (lane == 0) g = atomicAdd(p.queue, 1);
+    g = __builtin_amdgcn_readfirstlane(readlane_i32(g, 0));
+    if (g >= p.ngroups) break;
+    const GroupDev gd = p.groups[g];
+    const LayoutDev L = p.layouts[gd.layout];
+    for (int k = lane; k < L.S; k += 64) pos[k] = L.pos4[k];
+    wave_lds_sync();
+    PosPolicy<C> pol;
+    pol.init(pos, L, lane);
+    for (int i = 0; i < gd.count; ++i) run_trajectory<C, REC>(p, gd.first + i, L, pol, lane);
+    wave_lds_sync();
+  }
+}
+
+#define KMC_INST(C, W)                                                             \
   template __global__ void kmc_batch_fp32_kernel<C, W, false, false>(BatchParams); \
   template __global__ void kmc_batch_fp32_kernel<C, W, true, false>(BatchParams);  \
   template __global__ void kmc_batch_fp32_kernel<C, W, false, true>(BatchParams);  \
-  template __global__ void kmc_batch_fp32_kernel<C, W, true, true>(BatchParams);
+  template __global__ void kmc_batch_fp32_kernel<C, W, true, true>(BatchParams);   \
+  template __global__ void kmc_batch_pos_kernel<C, W, false>(BatchParams);         \
+  template __global__ void kmc_batch_pos_kernel<C, W, true>(BatchParams);
 KMC_INST(1, 1) KMC_INST(1, 2) KMC_INST(1, 4) KMC_INST(1, 8)
 KMC_INST(2, 1) KMC_INST(2, 2) KMC_INST(2, 4) KMC_INST(2, 8)
 KMC_INST(3, 1) KMC_INST(3, 2) KMC_INST(3, 4) KMC_INST(3, 8)
@@ -373,5 +480,7 @@ size_t batch_lds_bytes(bool bt, int S, int N) {
   if (bt) b += ((size_t)N * S + 64 * 3) * sizeof(float);
   return b;
 }
+
+size_t pos_lds_bytes(int C, int W) { return (size_t)W * 64 * C * sizeof(float4); }
 
 }  // namespace kmc

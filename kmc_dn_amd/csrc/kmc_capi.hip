@@ -24,9 +24,12 @@ template <bool REC, bool STAGE>
 __global__ void kmc_plain_fp64_kernel(PlainParams);
 template <int C, int W, bool REC, bool BT>
 __global__ void kmc_batch_fp32_kernel(BatchParams);
+template <int C, int W, bool REC>
+__global__ void kmc_batch_pos_kernel(BatchParams);
 __global__ void kmc_queue_reset_kernel(int32_t*);
 size_t plain_lds_bytes(int S, int N, bool stage);
 size_t batch_lds_bytes(bool bt, int S, int N);
+size_t pos_lds_bytes(int C, int W);
 
 namespace {
 
@@ -82,18 +85,19 @@ struct Layout {
   ~Layout() {
     for (DevBuf* b : eset_bufs) delete b;
   }
-  DevBuf tab32, tabB, ccol, coul64, tc64, ird64, invd64;
+  DevBuf tab32, tabB, ccol, coul64, tc64, ird64, invd64, pos4;
   LayoutDev dev{};
 };
 
 struct Batch {
   int64_t ntraj = 0;
+  int64_t capacity = 0;
+  int group_mode = -1;              // waves per group of the current groups (0: one per group)
   std::vector<TrajDev> host;        // in layout-grouped order
   std::vector<int64_t> perm;        // grouped position -> caller index
   int smax = 0, nmax = 0;
   DevBuf trajs, out, occ_time, traffic, queue, groups;
   int32_t ngroups = 0;
-  int last_W = -1;
   bool last_rec = false;
   bool has_occ_time = false;
   bool has_traffic = false;
@@ -321,6 +325,43 @@ int eset_create_locked(int32_t layout, int64_t count, const double* Ec, const do
   return KMC_OK;
 }
 
+// Positions form: verify the layout's tables are the reference closed forms of
+// these coordinates, then store float4 coordinates and the closed-form constants.
+int layout_set_positions_locked(Layout* L, double ab, const double* xyz) {
+  const int S = L->S, n = L->N;
+  std::vector<double> d((size_t)S * S), tc((size_t)S * S);
+  HIPOK(hipMemcpy(tc.data(), L->tc64.p, sizeof(double) * S * S, hipMemcpyDeviceToHost));
+  const double nu = L->dev.nu;
+  for (int i = 0; i < S; ++i)
+    for (int j = 0; j < S; ++j) {
+      const double dx = xyz[3 * i] - xyz[3 * j], dy = xyz[3 * i + 1] - xyz[3 * j + 1],
+                   dz = xyz[3 * i + 2] - xyz[3 * j + 2];
+      const double dd = std::sqrt(dx * dx + dy * dy + dz * dz);
+      if (i == j) continue;
+      if (i >= n && j >= n) continue;   // electrode pairs never hop
+      const double want = nu * std::exp(-2 * dd / ab);
+      const double got = tc[(size_t)i * S + j];
+      if (std::fabs(got - want) > 1e-9 * want + 1e-300)
+        return fail(KMC_ERR_ARGS, "transitions_constant[%d][%d]=%g is not nu*exp(-2d/ab)=%g for "
+                    "these positions", i, j, got, want);
+      if (i < n && j < n && !(dd > 0)) return fail(KMC_ERR_ARGS, "coincident acceptors %d, %d", i, j);
+    }
+  std::vector<float4> p4(S);
+  for (int i = 0; i < S; ++i)
+    p4[i] = make_float4((float)xyz[3 * i], (float)xyz[3 * i + 1], (float)xyz[3 * i + 2],
+                        i < n ? 1.f : 0.f);
+  int st = upload(L->pos4, p4);
+  if (st) return st;
+  const double scale2 = (double)L->dev.scale2;
+  L->dev.pos4 = L->pos4.as<float4>();
+  L->dev.pos_l0 = (float)(2.0 * std::log2(nu));
+  L->dev.pos_mkl = (float)(-2.0 * 1.4426950408889634 / ab);
+  L->dev.pos_kc = (float)(scale2 * L->dev.I0R);
+  L->dev.has_pos = 1;
+  eng().layouts_dirty = true;
+  return KMC_OK;
+}
+
 int layout_destroy_locked(int32_t layout) {
   Engine& E = eng();
   if (layout < 0 || layout >= (int)E.layouts.size() || !E.layouts[layout])
@@ -366,42 +407,37 @@ Batch* get_batch(int32_t id) {
   return E.batches[id];
 }
 
-int batch_create_locked(const kmc_traj* t, int64_t ntraj, int32_t* out) {
+// Validates descriptors and fills B->host / B->perm / B->smax / B->nmax.
+int batch_prepare(Batch* B, const kmc_traj* t, int64_t ntraj) {
   Engine& E = eng();
-  if (!t || ntraj <= 0 || !out) return fail(KMC_ERR_ARGS, "empty batch");
+  if (!t || ntraj <= 0) return fail(KMC_ERR_ARGS, "empty batch");
   if (ntraj > (1 << 30)) return fail(KMC_ERR_LIMIT, "too many trajectories");
-  Batch* B = new Batch();
-  B->ntraj = ntraj;
-  B->perm.resize(ntraj);
-  std::iota(B->perm.begin(), B->perm.end(), 0);
   for (int64_t i = 0; i < ntraj; ++i) {
     const kmc_traj& x = t[i];
     if (x.layout < 0 || x.layout >= (int)E.layouts.size() || !E.layouts[x.layout] || x.eset < 0 ||
-        x.eset >= (int)E.esets_host.size() || E.eset_layout[x.eset] != x.layout) {
-      delete B;
+        x.eset >= (int)E.esets_host.size() || E.eset_layout[x.eset] != x.layout)
       return fail(KMC_ERR_ARGS, "trajectory %lld: bad layout/energy-set pair (%d, %d)",
                   (long long)i, x.layout, x.eset);
-    }
-    if (x.hops < 0 || x.prehops < 0 || x.hops + x.prehops > (int64_t)INT32_MAX) {
-      delete B;
+    if (x.hops < 0 || x.prehops < 0 || x.hops + x.prehops > (int64_t)INT32_MAX)
       return fail(KMC_ERR_ARGS, "trajectory %lld: hops out of range", (long long)i);
-    }
     const int N = E.layouts[x.layout]->N;
     for (int w = 0; w < OCCW; ++w) {
       const int lo = w * 64;
       uint64_t valid = N <= lo ? 0 : (N - lo >= 64 ? ~0ull : ((1ull << (N - lo)) - 1));
-      if (x.occ[w] & ~valid) {
-        delete B;
+      if (x.occ[w] & ~valid)
         return fail(KMC_ERR_ARGS, "trajectory %lld: occupation bit beyond N=%d", (long long)i, N);
-      }
     }
   }
+  B->ntraj = ntraj;
+  B->perm.resize(ntraj);
+  std::iota(B->perm.begin(), B->perm.end(), 0);
   // group by layout, longest budgets first inside each layout
   std::stable_sort(B->perm.begin(), B->perm.end(), [&](int64_t a, int64_t b) {
     if (t[a].layout != t[b].layout) return t[a].layout < t[b].layout;
     return t[a].prehops + t[a].hops > t[b].prehops + t[b].hops;
   });
   B->host.resize(ntraj);
+  B->smax = B->nmax = 0;
   for (int64_t i = 0; i < ntraj; ++i) {
     const kmc_traj& x = t[B->perm[i]];
     TrajDev& d = B->host[i];
@@ -412,10 +448,29 @@ int batch_create_locked(const kmc_traj* t, int64_t ntraj, int32_t* out) {
     d.prehops = x.prehops;
     d.hops = x.hops;
     for (int w = 0; w < OCCW; ++w) d.occ[w] = x.occ[w];
+    d.h_begin = 0;
+    d.h_end = x.prehops + x.hops;
     B->smax = std::max(B->smax, E.layouts[x.layout]->S);
     B->nmax = std::max(B->nmax, E.layouts[x.layout]->N);
   }
-  int st;
+  B->group_mode = -1;
+  if (B->graph) {
+    (void)hipGraphExecDestroy(B->graph);
+    B->graph = nullptr;
+  }
+  return KMC_OK;
+}
+
+int batch_create_locked(const kmc_traj* t, int64_t ntraj, int32_t* out) {
+  Engine& E = eng();
+  if (!out) return fail(KMC_ERR_ARGS, "null batch id pointer");
+  Batch* B = new Batch();
+  int st = batch_prepare(B, t, ntraj);
+  if (st) {
+    delete B;
+    return st;
+  }
+  B->capacity = ntraj;
   if ((st = upload(B->trajs, B->host)) || (st = B->out.alloc(sizeof(TrajOut) * ntraj)) ||
       (st = B->queue.alloc(64))) {
     delete B;
@@ -440,8 +495,56 @@ int batch_create_locked(const kmc_traj* t, int64_t ntraj, int32_t* out) {
   return KMC_OK;
 }
 
+int batch_update_locked(Batch* B, const kmc_traj* t, int64_t ntraj) {
+  (void)hipStreamSynchronize(eng().stream);
+  int st = batch_prepare(B, t, ntraj);
+  if (st) return st;
+  if (ntraj > B->capacity) {
+    if ((st = B->out.alloc(sizeof(TrajOut) * ntraj))) return st;
+    if (B->has_occ_time && (st = B->occ_time.alloc(sizeof(double) * NMAX * ntraj))) return st;
+    B->has_traffic = false;
+    B->capacity = ntraj;
+    return upload(B->trajs, B->host);
+  }
+  HIPOK(hipMemcpy(B->trajs.p, B->host.data(), sizeof(TrajDev) * ntraj, hipMemcpyHostToDevice));
+  return KMC_OK;
+}
+
+int batch_set_state_locked(Batch* B, const int64_t* hb, const int64_t* he, const double* time,
+                           const int64_t* eo, const uint64_t* occ) {
+  Engine& E = eng();
+  (void)hipStreamSynchronize(E.stream);
+  if (hb || he) {
+    for (int64_t i = 0; i < B->ntraj; ++i) {
+      const int64_t j = B->perm[i];
+      TrajDev& d = B->host[i];
+      if (hb) d.h_begin = hb[j];
+      if (he) d.h_end = he[j];
+      if (d.h_begin < 0 || d.h_end < d.h_begin)
+        return fail(KMC_ERR_ARGS, "trajectory %lld: bad hop range", (long long)j);
+    }
+    HIPOK(hipMemcpy(B->trajs.p, B->host.data(), sizeof(TrajDev) * B->ntraj, hipMemcpyHostToDevice));
+  }
+  if (time || eo || occ) {
+    std::vector<TrajOut> o(B->ntraj);
+    HIPOK(hipMemcpy(o.data(), B->out.p, sizeof(TrajOut) * B->ntraj, hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < B->ntraj; ++i) {
+      const int64_t j = B->perm[i];
+      if (time) o[i].time = time[j];
+      if (eo)
+        for (int e = 0; e < PMAX; ++e) o[i].eo[e] = eo[j * PMAX + e];
+      if (occ)
+        for (int w = 0; w < OCCW; ++w) o[i].occ[w] = occ[j * OCCW + w];
+      o[i].status = KMC_OK;
+    }
+    HIPOK(hipMemcpy(B->out.p, o.data(), sizeof(TrajOut) * B->ntraj, hipMemcpyHostToDevice));
+  }
+  return KMC_OK;
+}
+
 // Groups of <= W consecutive same-layout trajectories, ordered by decreasing budget.
 int build_groups(Batch* B, int W) {
+  if (B->group_mode == W) return KMC_OK;
   std::vector<GroupDev> g;
   int64_t i = 0;
   while (i < B->ntraj) {
@@ -461,6 +564,7 @@ int build_groups(Batch* B, int W) {
     return x.prehops + x.hops > y.prehops + y.hops;
   });
   B->ngroups = (int32_t)g.size();
+  B->group_mode = W;
   return upload(B->groups, g);
 }
 
@@ -481,6 +585,18 @@ BatchKernel pick_batch_kernel(int C, int W, bool rec, bool bt) {
   return nullptr;
 }
 
+BatchKernel pick_pos_kernel(int C, int W, bool rec) {
+#define KMC_PICK(CC, WW)                                                             \
+  if (C == CC && W == WW)                                                            \
+    return rec ? (BatchKernel)kmc_batch_pos_kernel<CC, WW, true>                     \
+               : (BatchKernel)kmc_batch_pos_kernel<CC, WW, false>;
+  KMC_PICK(1, 1) KMC_PICK(1, 2) KMC_PICK(1, 4) KMC_PICK(1, 8)
+  KMC_PICK(2, 1) KMC_PICK(2, 2) KMC_PICK(2, 4) KMC_PICK(2, 8)
+  KMC_PICK(3, 1) KMC_PICK(3, 2) KMC_PICK(3, 4) KMC_PICK(3, 8)
+#undef KMC_PICK
+  return nullptr;
+}
+
 struct Fp32Launch {
   BatchKernel k = nullptr;
   int grid = 0, block = 0;
@@ -493,45 +609,53 @@ struct Fp32Launch {
 int prepare_fp32(Batch* B, const kmc_run_opts& o, Fp32Launch& L) {
   Engine& E = eng();
   const int C = (B->smax + 63) / 64;
+  const bool pos = o.table_form == 3;
   int W = o.waves_per_group;
-  if (W <= 0) W = C == 1 ? 4 : (C == 2 ? 2 : 1);
+  if (W <= 0) W = pos ? 4 : (C == 1 ? 4 : (C == 2 ? 2 : 1));
   if (W != 1 && W != 2 && W != 4 && W != 8) return fail(KMC_ERR_ARGS, "waves_per_group must be 1, 2, 4 or 8");
   const bool rec = (o.record & 1) != 0;
   const bool trf = (o.record & 2) != 0;
   const size_t lds_cap = (size_t)160 * 1024;
-  bool bt;
-  if (o.table_form == 1) bt = true;
-  else if (o.table_form == 2) bt = false;
-  else if (o.table_form == 0) bt = batch_lds_bytes(true, B->smax, B->nmax) <= lds_cap;
-  else return fail(KMC_ERR_ARGS, "table_form must be 0, 1 or 2");
-  BatchKernel k = pick_batch_kernel(C, W, rec, bt);
+  BatchKernel k = nullptr;
+  size_t lds = 0;
+  if (pos) {
+    for (const TrajDev& t : B->host)
+      if (!E.layouts[t.layout]->dev.has_pos)
+        return fail(KMC_ERR_ARGS, "layout %d has no positions (kmc_layout_set_positions)", t.layout);
+    k = pick_pos_kernel(C, W, rec);
+    lds = pos_lds_bytes(C, W);
+  } else {
+    bool bt;
+    if (o.table_form == 1) bt = true;
+    else if (o.table_form == 2) bt = false;
+    else if (o.table_form == 0) bt = batch_lds_bytes(true, B->smax, B->nmax) <= lds_cap;
+    else return fail(KMC_ERR_ARGS, "table_form must be 0, 1, 2 or 3");
+    k = pick_batch_kernel(C, W, rec, bt);
+    lds = batch_lds_bytes(bt, B->smax, B->nmax);
+  }
   if (!k) return fail(KMC_ERR_LIMIT, "no kernel for %d site chunks", C);
-  const size_t lds = batch_lds_bytes(bt, B->smax, B->nmax);
   if (lds > lds_cap)
     return fail(KMC_ERR_LIMIT, "LDS %zu bytes for S=%d exceeds 160 KiB", lds, B->smax);
   HIPOK(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  if (W != B->last_W) {
-    int st = build_groups(B, W);
-    if (st) return st;
-    B->last_W = W;
-  }
+  int st = build_groups(B, pos ? 1 : W);   // positions: each wave pulls one trajectory
+  if (st) return st;
   int per_cu = 0;
   HIPOK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k, 64 * W, lds));
   if (per_cu <= 0) return fail(KMC_ERR_LIMIT, "kernel does not fit on a CU (LDS %zu)", lds);
   int grid = per_cu * E.cu_count;
   if (o.grid_limit > 0) grid = std::min(grid, o.grid_limit);
-  grid = std::max(1, std::min(grid, (int)B->ngroups));
+  const int work_units = pos ? (B->ngroups + W - 1) / W : B->ngroups;
+  grid = std::max(1, std::min(grid, work_units));
   if (rec && !B->has_occ_time) {
-    int st = B->occ_time.alloc(sizeof(double) * NMAX * B->ntraj);
-    if (st) return st;
+    if ((st = B->occ_time.alloc(sizeof(double) * NMAX * B->capacity))) return st;
+    HIPOK(hipMemset(B->occ_time.p, 0, sizeof(double) * NMAX * B->capacity));
     B->has_occ_time = true;
   }
   const int64_t tstride = (int64_t)B->smax * B->smax;
   if (trf) {
     if (o.use_graph) return fail(KMC_ERR_ARGS, "traffic recording is not available with use_graph");
     if (!B->has_traffic) {
-      int st = B->traffic.alloc(sizeof(double) * tstride * B->ntraj);
-      if (st) return st;
+      if ((st = B->traffic.alloc(sizeof(double) * tstride * B->capacity))) return st;
       B->has_traffic = true;
     }
   }
@@ -548,7 +672,8 @@ int prepare_fp32(Batch* B, const kmc_run_opts& o, Fp32Launch& L) {
   p.traffic = trf ? B->traffic.as<double>() : nullptr;
   p.traffic_stride = tstride;
   p.post_hop_record = o.post_hop_record;
-  L.traffic_bytes = trf ? sizeof(double) * (size_t)tstride * B->ntraj : 0;
+  p.resume = o.resume ? 1 : 0;
+  L.traffic_bytes = (trf && !o.resume) ? sizeof(double) * (size_t)tstride * B->ntraj : 0;
   L.k = k;
   L.grid = grid;
   L.block = 64 * W;
@@ -557,7 +682,7 @@ int prepare_fp32(Batch* B, const kmc_run_opts& o, Fp32Launch& L) {
 }
 
 int launch_fp32(Fp32Launch& L, hipStream_t s) {
-  if (L.p.traffic) HIPOK(hipMemsetAsync(L.p.traffic, 0, L.traffic_bytes, s));
+  if (L.traffic_bytes) HIPOK(hipMemsetAsync(L.p.traffic, 0, L.traffic_bytes, s));
   int32_t* q = L.p.queue;
   void* qa[] = {&q};
   HIPOK(hipLaunchKernel((const void*)kmc_queue_reset_kernel, dim3(1), dim3(1), qa, 0, s));
@@ -610,6 +735,7 @@ int batch_run_locked(int32_t id, const kmc_run_opts* opts) {
   if (st) return st;
   hipStream_t s = E.stream;
   if (o.precision == KMC_FP64_DENSE) {
+    if (o.resume) return fail(KMC_ERR_UNSUPPORTED, "resume runs use the FP32 batched kernel");
     HIPOK(hipEventRecord(B->ev0, s));
     st = enqueue_fp64(B, o, s, nullptr, nullptr, nullptr, nullptr, nullptr);
     if (st) return st;
@@ -624,7 +750,7 @@ int batch_run_locked(int32_t id, const kmc_run_opts* opts) {
     const int64_t key = (int64_t)(o.waves_per_group & 0xFF) | ((int64_t)(o.record & 1) << 8) |
                         ((int64_t)(o.post_hop_record & 1) << 9) |
                         ((int64_t)(o.recompute_interval & 0xFFFFF) << 10) |
-                        ((int64_t)(o.table_form & 3) << 30);
+                        ((int64_t)(o.table_form & 3) << 30) | ((int64_t)(o.resume & 1) << 32);
     if (!B->graph || B->graph_key != key || o.grid_limit) {
       if (B->graph) {
         (void)hipGraphExecDestroy(B->graph);
@@ -715,6 +841,32 @@ int kmc_layout_create(int64_t N, int64_t P, double nu, double kT, double I_0, do
   if (st) return st;
   return layout_create_locked(N, P, nu, kT, I_0, R, distances, transitions_constant,
                               prune_threshold, layout_out);
+}
+
+int kmc_layout_set_positions(int32_t layout, double ab, const double* positions) {
+  Engine& E = eng();
+  std::lock_guard<std::mutex> lk(E.mu);
+  if (layout < 0 || layout >= (int)E.layouts.size() || !E.layouts[layout])
+    return fail(KMC_ERR_ARGS, "bad layout id %d", layout);
+  if (!positions || !(ab > 0)) return fail(KMC_ERR_ARGS, "need positions and ab > 0");
+  return layout_set_positions_locked(E.layouts[layout], ab, positions);
+}
+
+int kmc_batch_update(int32_t batch, const kmc_traj* trajs, int64_t ntraj) {
+  Engine& E = eng();
+  std::lock_guard<std::mutex> lk(E.mu);
+  Batch* B = get_batch(batch);
+  if (!B) return fail(KMC_ERR_ARGS, "bad batch id %d", batch);
+  return batch_update_locked(B, trajs, ntraj);
+}
+
+int kmc_batch_set_state(int32_t batch, const int64_t* h_begin, const int64_t* h_end,
+                        const double* time, const int64_t* eo, const uint64_t* occ) {
+  Engine& E = eng();
+  std::lock_guard<std::mutex> lk(E.mu);
+  Batch* B = get_batch(batch);
+  if (!B) return fail(KMC_ERR_ARGS, "bad batch id %d", batch);
+  return batch_set_state_locked(B, h_begin, h_end, time, eo, occ);
 }
 
 int kmc_layout_destroy(int32_t layout) {

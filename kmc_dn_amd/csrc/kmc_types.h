@@ -36,6 +36,11 @@ struct LayoutDev {
   const float2* tabB;
   const float* ccol;
   const double* coul64;   // [N*N] I_0*R/d for acceptor pairs (symmetric, 0 diagonal)
+  // Positions policy: site coordinates (w = 1 acceptor, 0 electrode) and the
+  // closed forms lt = pos_l0 + pos_mkl * d, c = pos_kc / d (null if absent).
+  const float4* pos4;
+  float pos_l0, pos_mkl, pos_kc;
+  int32_t has_pos;
   // FP64 dense kernel (reference arithmetic).
   const double* tc64;     // [S*S] transitions_constant
   const double* ird64;    // [S*S] (I_0*R)/d[i][j] (acceptor pairs; 0 elsewhere)
@@ -55,6 +60,8 @@ struct TrajDev {
   int64_t prehops;    // hops run before tallies start (not counted)
   int64_t hops;       // counted hops
   uint64_t occ[OCCW]; // initial occupation bitmask
+  int64_t h_begin;    // absolute hop range of this launch (with BatchParams::resume)
+  int64_t h_end;
 };
 
 struct TrajOut {
@@ -86,7 +93,7 @@ struct BatchParams {
   double* traffic;              // [ntraj * traffic_stride] net traffic, or null
   int64_t traffic_stride;       // >= S*S of every layout in the batch
   int32_t post_hop_record;      // 1: numba convention (post-hop state)
-  int32_t pad;
+  int32_t resume;               // 1: run [h_begin, h_end) from the state in `out`
 };
 
 struct PlainParams {

@@ -84,6 +84,12 @@ class Layout:
                                          C.byref(first)))
         return np.arange(first.value, first.value + E.shape[0], dtype=np.int32)
 
+    def set_positions(self, ab: float, positions) -> "Layout":
+        """Site coordinates (S, 3) for the positions form (table_form=3)."""
+        xyz = np.ascontiguousarray(positions, dtype=np.float64).reshape(self.S, 3)
+        L.check(L.load().kmc_layout_set_positions(self.id, float(ab), L.ptr(xyz, L._dp)))
+        return self
+
     @staticmethod
     def drop_energy_sets(ids) -> None:
         for i in np.atleast_1d(ids):
@@ -138,13 +144,36 @@ class Batch:
     def run(self, precision: int = L.FP32_BATCHED, waves_per_group: int = 0,
             recompute_interval: int = 0, record: bool = False, post_hop_record: bool = False,
             grid_limit: int = 0, use_graph: bool = False, traffic: bool = False,
-            table_form: int = 0) -> None:
+            table_form: int = 0, resume: bool = False) -> None:
         rec = (1 if record else 0) | (2 if traffic else 0)
         o = L.kmc_run_opts(int(precision), int(waves_per_group), int(recompute_interval),
                            rec, int(post_hop_record), int(grid_limit), int(use_graph),
-                           int(table_form))
+                           int(table_form), int(bool(resume)), 0)
         L.check(L.load().kmc_batch_run(self.id, C.byref(o)))
         self.recorded = bool(record)
+
+    def update(self, trajs: np.ndarray) -> None:
+        """Replace the descriptors (reusing device buffers when they fit)."""
+        trajs = np.ascontiguousarray(trajs, dtype=TRAJ_DTYPE)
+        L.check(L.load().kmc_batch_update(self.id, trajs.ctypes.data_as(C.POINTER(L.kmc_traj)),
+                                          trajs.shape[0]))
+        self.n = trajs.shape[0]
+
+    def set_state(self, h_begin=None, h_end=None, time=None, eo=None, occ_bits=None) -> None:
+        """Hop ranges and start state of the next resume run (caller order)."""
+        def a(x, dt, shape=None):
+            if x is None:
+                return None
+            x = np.ascontiguousarray(x, dtype=dt)
+            return x.reshape(shape) if shape else x
+        hb = a(h_begin, np.int64)
+        he = a(h_end, np.int64)
+        tm = a(time, np.float64)
+        e = a(eo, np.int64, (self.n, L.PMAX))
+        o = a(occ_bits, np.uint64, (self.n, L.OCCW))
+        L.check(L.load().kmc_batch_set_state(self.id, L.ptr(hb, L._i64p), L.ptr(he, L._i64p),
+                                             L.ptr(tm, L._dp), L.ptr(e, L._i64p),
+                                             L.ptr(o, L._u64p)))
 
     def sync(self) -> float:
         ms = C.c_double()

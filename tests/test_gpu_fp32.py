@@ -132,3 +132,61 @@ def test_table_forms_agree(gpu_engine):
     ca, _ = _ensemble(net, _lib.FP32_BATCHED, 1024, 10000, table_form=1)
     cb, _ = _ensemble(net, _lib.FP32_BATCHED, 1024, 10000, table_form=2)
     _compare(ca, cb, "table {lt,lt+c} vs {lt,c}")
+
+
+def _positions(net):
+    from kmc_dn_amd import tables as T
+    return T.site_positions(net["acceptors"], net["electrodes"])
+
+
+def test_positions_form_matches_table_form(gpu_engine):
+    net = random_network(21, 50, 8, I_0=30.0, a=0.25, vmax=2.0)
+    N, P = 50, 8
+    lay = engine.Layout(N, P, 1.0, 1.0, net["I_0"], net["R"], net["distances"],
+                        net["transitions_constant"])
+    lay.set_positions(0.25 * net["R"], _positions(net))
+    es = lay.energy_sets(net["E_constant"], net["site_energies"][N:])
+    b = engine.Batch(engine.make_trajs(1024, lay.id, es[0], np.arange(1024), 10000,
+                                       net["occupation"], prehops=1000))
+    out = {}
+    for form in (1, 3):
+        b.run(precision=_lib.FP32_BATCHED, table_form=form)
+        b.sync()
+        r = b.results()
+        assert (r.status == 0).all()
+        out[form] = r.currents(P)
+    _compare(out[3], out[1], "positions vs {lt,lt+c} table")
+
+
+def test_resume_in_chunks_is_bit_exact(gpu_engine):
+    net = random_network(22, 30, 8, I_0=30.0, a=0.25, vmax=2.0)
+    N, P = 30, 8
+    K, hops, pre = 64, 9000, 500
+    for form in (1, 3):
+        lay = engine.Layout(N, P, 1.0, 1.0, net["I_0"], net["R"], net["distances"],
+                            net["transitions_constant"])
+        lay.set_positions(0.25 * net["R"], _positions(net))
+        es = lay.energy_sets(net["E_constant"], net["site_energies"][N:])
+        t = engine.make_trajs(K, lay.id, es[0], np.arange(K), hops, net["occupation"], prehops=pre)
+        whole = engine.Batch(t)
+        whole.run(precision=_lib.FP32_BATCHED, table_form=form, record=True)
+        whole.sync()
+        ref = whole.results()
+        b = engine.Batch(t)
+        total = hops + pre
+        done = np.zeros(K, np.int64)
+        chunk = 2048                           # a multiple of the recompute interval
+        first = True
+        while (done < total).any():
+            end = np.minimum(done + chunk, total)
+            b.set_state(h_begin=done, h_end=end)
+            b.run(precision=_lib.FP32_BATCHED, table_form=form, record=True, resume=True)
+            b.sync()
+            done = end
+            first = False
+        r = b.results()
+        assert not first
+        assert np.array_equal(r.eo, ref.eo) and np.array_equal(r.time, ref.time), form
+        assert np.array_equal(r.occ_bits, ref.occ_bits)
+        assert np.array_equal(r.occ_time, ref.occ_time)
+        assert (r.hops_done == hops).all()
