@@ -32,9 +32,9 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 METRIC = "hops/sec, 4096-traj N=30 ensemble vs plain; p50/p99 latency of served IV jobs"
-# MI355X: 256 CUs at up to 2.4 GHz; LDS 160 KiB per CU delivering 128 bytes per clock
-# to the SIMDs (ds_read_b128 path) -> 256 * 128 * 2.4e9 B/s.
-LDS_PEAK_GBS = 256 * 128 * 2.4
+# MI355X: 256 CUs at up to 2.4 GHz; CDNA4's LDS (160 KiB per CU) delivers 256 bytes per
+# clock per CU (twice CDNA3's 128) -> 256 * 256 * 2.4e9 B/s = 157.3 TB/s.
+LDS_PEAK_GBS = 256 * 256 * 2.4
 # HBM bytes per launch of the batched kernel measured with rocprofv3 --pmc FETCH_SIZE
 # and WRITE_SIZE on this workload, scaled by the copy-kernel calibration factor
 # (profiles/kmc_batch_fp32_c1_hbm_traffic.md); None until measured.
@@ -115,6 +115,80 @@ def iv_leg(lay, args) -> dict:
             "current_e0_first_last": [float(mean[0, 0]), float(mean[0, -1])]}
 
 
+def uneven_leg(args) -> dict:
+    """2048 distinct 3D networks (N in [10,120], 4-12 electrodes, log-uniform budgets
+    1e4..1e6) in one graph-captured launch under the autotuned bucket plan."""
+    from kmc_dn_amd import uneven, workloads
+    t0 = time.perf_counter()
+    ub = workloads.uneven_batch(seed=0, n=args.uneven_n)
+    dev = uneven.upload(ub)
+    setup = time.perf_counter() - t0
+    try:
+        t0 = time.perf_counter()
+        from kmc_dn_amd import autotune
+        plan = autotune.plan_for(dev.trajs, dev.sizes)
+        tune_s = time.perf_counter() - t0
+        run = uneven.run(dev, plan=plan, use_graph=True)
+        uneven.check_invariants(dev, run.result)
+    finally:
+        dev.close()
+    S = np.array([n.N + n.P for n in ub.nets])
+    N = np.array([n.N for n in ub.nets])
+    P = np.array([n.P for n in ub.nets])
+    bph = np.array([lds_bytes_per_hop(int(a), int(b)) for a, b in zip(N, P)])
+    achieved = float(np.sum(bph * ub.hops)) / (run.kernel_ms * 1e-3) / 1e9
+    longest = int(np.argmax(ub.hops))
+    return {"value": run.hops_per_s, "unit": "hops/s", "kernel_ms": run.kernel_ms,
+            "hops": run.hops, "networks": args.uneven_n, "plan": [list(p) for p in run.plan],
+            "S_range": [int(S.min()), int(S.max())],
+            "longest_budget": int(ub.hops[longest]), "longest_S": int(S[longest]),
+            "lds_roofline_frac": achieved / LDS_PEAK_GBS,
+            "setup_s": setup, "tune_s": tune_s,
+            "workload": "2048 seeded 3D networks in 1x1x0.2, N~U{10..120}, M=round(N/10), "
+                        "P~U{4..12} side-face electrodes, V~U(-1,1), hops log-uniform [1e4,1e6]; "
+                        "one trajectory per network; bucket kernels forked in one hipGraph"}
+
+
+def serving_leg(args) -> dict:
+    """Poisson request streams over 64 resident layouts, served by continuous
+    batching; beside them the one-launch-per-request baseline."""
+    from kmc_dn_amd import serving, workloads
+    nets, bases = workloads.resident_layouts(seed=0, n=64, N_range=(20, 100))
+    srv = serving.Server(nets, bases, chunk_hops=args.serve_chunk)
+    out = {"plan": [list(p) for p in srv.plan], "chunk_hops": args.serve_chunk,
+           "workload": "64 resident 2D layouts N~U{20..100}, 8 electrodes; requests: one "
+                       "layout, V~U(-1.5,1.5)^8, hops lognormal(median 5e4, sigma 1), Poisson "
+                       "arrivals; latency = completion - arrival"}
+    try:
+        for rate in args.serve_rates:
+            reqs = workloads.request_stream(rate, int(rate * args.serve_seconds), nets, seed=rate)
+            sreq = [serving.ServeRequest(r.rid, r.t_arrive, r.layout, r.V, r.hops) for r in reqs]
+            hops = int(sum(r.hops for r in sreq))
+            r0, h0 = srv.rounds, srv.hops_done
+            done, wall = serving.serve_stream(srv, sreq, realtime=True, max_wall=300)
+            d = serving.latency_summary(done, wall, hops)
+            d.update(offered_req_per_s=rate, requests=len(sreq), rounds=srv.rounds - r0,
+                     offered_hops_per_s=hops / sreq[-1].t_arrive,
+                     kernel_ms_total=float(np.sum(srv.round_ms[r0:])),
+                     host_s={k: round(v, 3) for k, v in srv.host_s.items()})
+            for k in srv.host_s:
+                srv.host_s[k] = 0.0
+            assert d["failed"] == 0 and srv.hops_done - h0 == hops
+            out[f"{rate}/s"] = d
+        reqs = workloads.request_stream(args.serve_rates[0], args.serve_baseline_n, nets,
+                                        seed=args.serve_rates[0])
+        sreq = [serving.ServeRequest(r.rid, r.t_arrive, r.layout, r.V, r.hops) for r in reqs]
+        done, wall = serving.serve_one_launch_per_request(srv, sreq, realtime=True)
+        d = serving.latency_summary(done, wall, int(sum(r.hops for r in sreq)))
+        d.update(offered_req_per_s=args.serve_rates[0], requests=len(sreq),
+                 sample=f"first {len(sreq)} requests of the {args.serve_rates[0]}/s stream, "
+                        "each launched alone on one stream as it arrives")
+        out["one_launch_per_request"] = d
+    finally:
+        srv.close()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -132,6 +206,13 @@ def main() -> int:
     ap.add_argument("--iv-replicas", type=int, default=64)
     ap.add_argument("--iv-hops", type=int, default=200_000)
     ap.add_argument("--no-iv", action="store_true")
+    ap.add_argument("--uneven-n", type=int, default=2048)
+    ap.add_argument("--no-uneven", action="store_true")
+    ap.add_argument("--serve-rates", type=int, nargs="+", default=[200, 1000, 5000])
+    ap.add_argument("--serve-seconds", type=float, default=3.0)
+    ap.add_argument("--serve-chunk", type=int, default=8192)
+    ap.add_argument("--serve-baseline-n", type=int, default=48)
+    ap.add_argument("--no-serve", action="store_true")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -223,6 +304,10 @@ def main() -> int:
     }
     if not args.no_iv:
         out["iv_sweep"] = iv_leg(lay, args)
+    if not args.no_uneven:
+        out["uneven_batch"] = uneven_leg(args)
+    if not args.no_serve:
+        out["serving"] = serving_leg(args)
     if not args.no_cpu_baseline:
         numba, go = cpu_baselines(ens, args.cpu_seconds)
         out["cpu_baseline"] = numba

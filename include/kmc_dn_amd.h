@@ -107,6 +107,17 @@ int kmc_eset_create(int32_t layout, int64_t count, const double* E_constant, con
 /* Frees one energy set (its memory is released with the last set of its upload). */
 int kmc_eset_destroy(int32_t eset);
 
+/* Overwrites one energy set in place (E_constant N, V P doubles; no allocation).
+ * Not ordered against running batches: rewrite only sets no queued run reads.
+ * The reference recomputes E_constant on the host per voltage vector
+ * (kmc_dopant_networks.py:865-899, update_V :798-821); this keeps it resident. */
+int kmc_eset_write(int32_t eset, const double* E_constant, const double* V);
+
+/* Overwrites n energy sets in one upload, ordered on the engine stream before
+ * any later batch run: set ids[i] takes E_constant from E[i*KMC_NMAX ...] (its
+ * layout's N values) and V from V[i*KMC_PMAX ...] (P values). */
+int kmc_eset_write_many(int64_t n, const int32_t* ids, const double* E, const double* V);
+
 typedef struct kmc_traj {
   int32_t layout;             /* from kmc_layout_create */
   int32_t eset;               /* from kmc_eset_create (same layout) */
@@ -132,7 +143,9 @@ typedef struct kmc_run_opts {
                                  kmc_layout_set_positions on every layout) */
   int32_t resume;             /* FP32: run each trajectory's [h_begin, h_end) set by
                                  kmc_batch_set_state, continuing from that state */
-  int32_t pad;
+  int32_t team;               /* FP32: waves per trajectory, 1 (0) solo, 2 or 4: the
+                                 walked set of each hop is split over the team (lower
+                                 hop latency for batches that do not fill the GPU) */
 } kmc_run_opts;
 
 /* Uploads trajectory descriptors and allocates device outputs. */
@@ -146,6 +159,28 @@ int kmc_batch_update(int32_t batch, const kmc_traj* trajs, int64_t ntraj);
  * starts at a multiple of the recompute interval continues bit-exactly. */
 int kmc_batch_set_state(int32_t batch, const int64_t* h_begin, const int64_t* h_end,
                         const double* time, const int64_t* eo, const uint64_t* occ);
+/* Bucket launch plan for batches of uneven networks (FP32 batched path).
+ * Each trajectory goes to the first bucket (in plan order) whose s_max >= its
+ * layout's S and whose min_hops <= its budget (prehops + hops); every
+ * trajectory must match one.  A bucket runs its trajectories in its own
+ * work-queue kernel with its own pair-term form, workgroup size and team size;
+ * the bucket kernels are launched as parallel branches of one stream fork/join
+ * (one graph when use_graph is set).  A set plan overrides table_form,
+ * waves_per_group and team of kmc_run_opts; nbuckets = 0 removes it.  No
+ * reference counterpart: the reference runs one network per call
+ * (simulationWrapper.go:83-169). */
+#define KMC_MAX_BUCKETS 8
+typedef struct kmc_bucket {
+  int32_t s_max;              /* largest S of this bucket */
+  int32_t table_form;         /* 1, 2 or 3 as in kmc_run_opts */
+  int32_t waves_per_group;    /* solo kernels: 1, 2, 4 or 8 trajectories per workgroup */
+  int32_t team;               /* waves per trajectory: 1 (solo), 2 or 4 */
+  int64_t min_hops;           /* smallest budget this bucket takes */
+  int32_t grid;               /* resident workgroups; 0 = as many as fit */
+  int32_t pad;
+} kmc_bucket;
+int kmc_batch_set_buckets(int32_t batch, const kmc_bucket* buckets, int32_t nbuckets);
+
 /* Enqueues the batch on the engine stream (asynchronous). */
 int kmc_batch_run(int32_t batch, const kmc_run_opts* opts);
 /* Waits for the last run; elapsed_ms (optional) = HIP-event time of its kernels. */

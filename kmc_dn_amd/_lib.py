@@ -39,7 +39,13 @@ class kmc_run_opts(C.Structure):
                 ("recompute_interval", C.c_int32), ("record", C.c_int32),
                 ("post_hop_record", C.c_int32), ("grid_limit", C.c_int32),
                 ("use_graph", C.c_int32), ("table_form", C.c_int32),
-                ("resume", C.c_int32), ("pad", C.c_int32)]
+                ("resume", C.c_int32), ("team", C.c_int32)]
+
+
+class kmc_bucket(C.Structure):
+    _fields_ = [("s_max", C.c_int32), ("table_form", C.c_int32),
+                ("waves_per_group", C.c_int32), ("team", C.c_int32),
+                ("min_hops", C.c_int64), ("grid", C.c_int32), ("pad", C.c_int32)]
 
 
 class GoSlice(C.Structure):
@@ -73,6 +79,9 @@ SIGNATURES = {
     "kmc_batch_set_state": (C.c_int, [_i32, _i64p, _i64p, _dp, _i64p, _u64p]),
     "kmc_eset_create": (C.c_int, [_i32, _i64, _dp, _dp, _i32p]),
     "kmc_eset_destroy": (C.c_int, [_i32]),
+    "kmc_eset_write": (C.c_int, [_i32, _dp, _dp]),
+    "kmc_eset_write_many": (C.c_int, [_i64, _i32p, _dp, _dp]),
+    "kmc_batch_set_buckets": (C.c_int, [_i32, C.POINTER(kmc_bucket), _i32]),
     "kmc_batch_create": (C.c_int, [C.POINTER(kmc_traj), _i64, _i32p]),
     "kmc_batch_run": (C.c_int, [_i32, C.POINTER(kmc_run_opts)]),
     "kmc_batch_sync": (C.c_int, [_i32, _dp]),

@@ -1,0 +1,255 @@
+"""Autotuner of the FP32 batched kernel's launch shape per network-size bucket.
+
+For every size bucket (S <= 64, <= 128, <= 192: one, two or three 64-site
+chunks per lane) it times the candidate pair-term forms
+
+  1  {lt, lt+c} pair table + c table in LDS, shared by the W waves of a workgroup
+  2  {lt, c} pair table in LDS
+  3  site positions in LDS, pair terms computed (no table)
+
+times the candidate workgroup sizes W in {1, 2, 4, 8} on a sample of the
+bucket's own trajectories (their layouts, energy sets and seeds, a short hop
+budget), and keeps the fastest.  Forms whose LDS footprint does not fit 160 KiB
+are skipped.  Results are cached in a JSON file keyed by device, bucket and the
+trajectories-per-layout class of the sample, so later runs of a similar batch
+reuse them without timing.
+
+No reference counterpart: the reference runs one network per call
+(goSimulation/simulationWrapper.go:83-169, kmc_dopant_networks.py:33-135).
+"""
+from __future__ import annotations
+
+import json
+import math
+import os
+
+import numpy as np
+
+from . import _lib as L
+from . import engine
+
+BUCKET_EDGES = (64, 128, 192)
+FORMS = (1, 2, 3)
+WAVES = (1, 2, 4, 8)
+LDS_BYTES = 160 * 1024
+CACHE_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "autotune_cache.json")
+
+
+def table_lds_bytes(form: int, S: int, N: int) -> int:
+    """LDS bytes of a table-form workgroup for S sites, N acceptors."""
+    b = (S * S + 192) * 8
+    if form == 1:
+        b += (N * S + 192) * 4
+    return b
+
+
+def bucket_of(S: int, edges=BUCKET_EDGES) -> int:
+    for b, e in enumerate(edges):
+        if S <= e:
+            return b
+    raise ValueError(f"S={S} above the largest bucket {edges[-1]}")
+
+
+def candidates(smax: int, nmax: int, has_pos: bool):
+    for form in FORMS:
+        if form == 3 and not has_pos:
+            continue
+        if form != 3 and table_lds_bytes(form, smax, nmax) > LDS_BYTES:
+            continue
+        for w in WAVES:
+            yield form, w
+
+
+def _load_cache(path: str) -> dict:
+    try:
+        with open(path) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
+
+
+def _save_cache(path: str, cache: dict) -> None:
+    tmp = path + ".tmp"
+    try:
+        with open(tmp, "w") as f:
+            json.dump(cache, f, indent=1, sort_keys=True)
+        os.replace(tmp, path)
+    except OSError:
+        pass   # a read-only tree just skips caching
+
+
+def cache_key(device: str, edge: int, traj_per_layout: float) -> str:
+    return f"{device}|S<={edge}|tpl{int(round(math.log2(max(1.0, traj_per_layout))))}"
+
+
+def time_candidate(trajs: np.ndarray, edge: int, form: int, w: int, reps: int = 2) -> float:
+    """Hops per second of one candidate on `trajs` (best of `reps` timed runs)."""
+    b = engine.Batch(trajs)
+    try:
+        b.set_buckets([(edge, form, w, 0)])
+        b.run(precision=L.FP32_BATCHED)
+        b.sync()
+        best = math.inf
+        for _ in range(reps):
+            b.run(precision=L.FP32_BATCHED)
+            best = min(best, b.sync())
+        r = b.results()
+        if (r.status != 0).any():
+            raise L.KmcError(L.ERR_ZERO_RATE, "a tuning trajectory stopped")
+        hops = float(np.sum(trajs["hops"] + trajs["prehops"]))
+        return hops / (best * 1e-3)
+    finally:
+        b.close()
+
+
+def time_latency(traj: np.ndarray, form: int, team: int, hops: int = 4096) -> float:
+    """Seconds per hop of one trajectory running alone (the hop latency)."""
+    t = traj.copy().reshape(1)
+    t["prehops"] = 0
+    t["hops"] = hops
+    b = engine.Batch(t)
+    try:
+        b.run(precision=L.FP32_BATCHED, table_form=form, team=team, waves_per_group=1)
+        b.sync()
+        b.run(precision=L.FP32_BATCHED, table_form=form, team=team, waves_per_group=1)
+        return b.sync() * 1e-3 / hops
+    finally:
+        b.close()
+
+
+def tune_latency(trajs: np.ndarray, sizes: dict, has_pos: bool = True, edges=BUCKET_EDGES,
+                 cache_path: str | None = CACHE_PATH, retune: bool = False, log=None) -> dict:
+    """Per size bucket: the fastest (form, team) for one trajectory alone and its
+    seconds per hop, plus the solo latency of every fitting form, measured on the
+    bucket's largest network.  Returns {edge: {"form", "team", "lat", "solo": {form: lat}}}."""
+    device = engine.device_info()["name"]
+    cache = _load_cache(cache_path) if cache_path else {}
+    lay = trajs["layout"]
+    S_of = np.array([sizes[int(x)][0] for x in lay])
+    N_of = np.array([sizes[int(x)][1] for x in lay])
+    out, changed = {}, False
+    for b, edge in enumerate(edges):
+        idx = np.nonzero(np.array([bucket_of(int(x), edges) for x in S_of]) == b)[0]
+        if idx.size == 0:
+            continue
+        big = idx[np.argmax(S_of[idx])]
+        key = f"{device}|S<={edge}|latency|S{int(S_of[big])}"
+        if key in cache and not retune:
+            out[edge] = cache[key]
+            continue
+        res = {}
+        for form in FORMS:
+            if form == 3 and not has_pos:
+                continue
+            if form != 3 and table_lds_bytes(form, int(S_of[big]), int(N_of[big])) + \
+                    2 * 4 * 3 * 64 * 4 > LDS_BYTES:
+                continue
+            for team in (1, 2, 4):
+                res[(form, team)] = time_latency(trajs[big], form, team)
+                if log:
+                    log(f"  latency S={int(S_of[big])}: form {form} team {team}: "
+                        f"{res[(form, team)] * 1e6:.2f} us/hop")
+        (form, team), lat = min(res.items(), key=lambda kv: kv[1])
+        entry = {"form": form, "team": team, "lat": lat,
+                 "solo": {str(f): v for (f, t), v in res.items() if t == 1}}
+        cache[key] = entry
+        out[edge] = entry
+        changed = True
+    if changed and cache_path:
+        _save_cache(cache_path, cache)
+    return out
+
+
+def plan_for(trajs: np.ndarray, sizes: dict, has_pos: bool = True, edges=BUCKET_EDGES,
+             cache_path: str | None = CACHE_PATH, team_min_hops: dict | None = None,
+             log=None) -> list:
+    """Launch plan for a batch: per size bucket a solo bucket tuned for throughput
+    and, in front of it, a team bucket tuned for hop latency that takes the
+    trajectories whose solo run would outlast the batch's throughput-bound time.
+
+    team_min_hops: {edge: budget} to fix the team threshold per bucket instead
+    (serving: a property of the request, not of the current mix)."""
+    solo = tune(trajs, sizes, has_pos=has_pos, edges=edges, cache_path=cache_path, log=log)
+    lat = tune_latency(trajs, sizes, has_pos=has_pos, edges=edges, cache_path=cache_path, log=log)
+    cache = _load_cache(cache_path) if cache_path else {}
+    device = engine.device_info()["name"]
+    lay = trajs["layout"]
+    S_of = np.array([sizes[int(x)][0] for x in lay])
+    budget = (trajs["hops"] + trajs["prehops"]).astype(np.float64)
+    bucket = np.array([bucket_of(int(s), edges) for s in S_of])
+    # throughput-bound time: the buckets' work at their tuned rates, run side by side
+    t_thr = 0.0
+    for edge, form, w, _ in solo:
+        b = edges.index(edge)
+        sel = bucket == b
+        tpl = sel.sum() / max(1, len(np.unique(lay[sel])))
+        rate = cache.get(cache_key(device, edge, tpl), {}).get("hops_per_s")
+        if rate:
+            t_thr = max(t_thr, float(budget[sel].sum()) / rate)
+    plan = []
+    for edge, form, w, _ in solo:
+        b = edges.index(edge)
+        lt = lat.get(edge)
+        if lt and lt["team"] > 1:
+            if team_min_hops is not None:
+                thr = team_min_hops.get(edge)
+            else:
+                solo_lat = lt["solo"].get(str(form), lt["lat"])
+                thr = int(math.ceil(t_thr / solo_lat)) if t_thr > 0 else None
+                # only worth a bucket when a trajectory of this size exceeds it
+                sel = bucket == b
+                if thr is not None and not (budget[sel] >= thr).any():
+                    thr = None
+            if thr is not None:
+                plan.append((edge, int(lt["form"]), 1, int(lt["team"]), int(thr)))
+        plan.append((edge, form, w, 1, 0))
+    return plan
+
+
+def tune(trajs: np.ndarray, sizes: dict, has_pos: bool = True, sample: int = 1024,
+         sample_hops: int = 2048, edges=BUCKET_EDGES, cache_path: str | None = CACHE_PATH,
+         retune: bool = False, log=None) -> list:
+    """Launch plan [(s_max, form, W, 0), ...] for the batch `trajs`.
+
+    sizes: layout id -> (S, N).  Every bucket that holds trajectories is timed on
+    up to `sample` of them with `sample_hops` hops each (or read from the cache).
+    """
+    device = engine.device_info()["name"]
+    cache = _load_cache(cache_path) if cache_path else {}
+    lay = trajs["layout"]
+    S_of = np.array([sizes[int(x)][0] for x in lay])
+    N_of = np.array([sizes[int(x)][1] for x in lay])
+    bucket = np.array([bucket_of(int(s), edges) for s in S_of])
+    plan = []
+    changed = False
+    for b, edge in enumerate(edges):
+        idx = np.nonzero(bucket == b)[0]
+        if idx.size == 0:
+            continue
+        tpl = idx.size / max(1, len(np.unique(lay[idx])))
+        key = cache_key(device, edge, tpl)
+        hit = cache.get(key)
+        if hit and not retune:
+            plan.append((edge, int(hit["form"]), int(hit["waves"]), 1))
+            continue
+        rng = np.random.RandomState(b)
+        pick = idx if idx.size <= sample else np.sort(rng.choice(idx, sample, replace=False))
+        t = trajs[pick].copy()
+        t["prehops"] = 0
+        t["hops"] = sample_hops
+        smax, nmax = int(S_of[pick].max()), int(N_of[pick].max())
+        # the bucket's own largest network fixes the LDS footprint of table forms
+        smax_all, nmax_all = int(S_of[idx].max()), int(N_of[idx].max())
+        scores = {}
+        for form, w in candidates(max(smax, smax_all), max(nmax, nmax_all), has_pos):
+            scores[(form, w)] = time_candidate(t, edge, form, w)
+            if log:
+                log(f"  bucket S<={edge}: form {form} W={w}: {scores[(form, w)]:.3e} hops/s")
+        (form, w), rate = max(scores.items(), key=lambda kv: kv[1])
+        cache[key] = {"form": form, "waves": w, "hops_per_s": rate,
+                      "scores": {f"{f}/{ww}": v for (f, ww), v in scores.items()}}
+        changed = True
+        plan.append((edge, form, w, 1))
+    if changed and cache_path:
+        _save_cache(cache_path, cache)
+    return plan

@@ -15,6 +15,7 @@ import shutil
 import subprocess
 import sys
 import tempfile
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
@@ -30,6 +31,7 @@ COMMON = [
 SOURCES = {
     "kmc_plain.hip": ["-ffp-contract=off"],
     "kmc_batch.hip": [],
+    "kmc_team.hip": [],
     "kmc_capi.hip": [],
     "kmc_probe.hip": [],
     "kmc_goshim.cpp": [],
@@ -46,14 +48,20 @@ def hipcc() -> str:
 def build(verbose: bool = False) -> str:
     cc = hipcc()
     with tempfile.TemporaryDirectory(prefix="kmc_build_") as tmp:
-        objs = []
+        cmds, objs = [], []
         for src, extra in SOURCES.items():
             obj = os.path.join(tmp, os.path.splitext(src)[0] + ".o")
-            cmd = [cc, *COMMON, *extra, "-c", os.path.join(CSRC, src), "-o", obj]
+            cmds.append([cc, *COMMON, *extra, "-c", os.path.join(CSRC, src), "-o", obj])
+            objs.append(obj)
+
+        def compile_one(cmd):
             if verbose:
                 print(" ".join(cmd), flush=True)
             subprocess.run(cmd, check=True)
-            objs.append(obj)
+
+        jobs = max(1, min(len(cmds), int(os.environ.get("MAX_JOBS", "4"))))
+        with ThreadPoolExecutor(jobs) as ex:
+            list(ex.map(compile_one, cmds))
         tmp_lib = LIB + ".tmp"
         cmd = [cc, *COMMON, "-shared", *objs, "-o", tmp_lib]
         if verbose:

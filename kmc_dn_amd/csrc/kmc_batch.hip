@@ -1,402 +1,11 @@
-// FP32-rate / FP64-accumulate batched hop loop (one wave per trajectory).
+// Solo kernels of the FP32 batched path: one wave per trajectory.
 //
-// Same model as _simulate_discrete_record (kmc_dopant_networks.py:33-135) and
-// Go's simulate (goSimulation/simulation.go:194-325), organised for CDNA4:
-//
-//  * Each lane owns sites s = lane + 64*c (c < C).  Site energies live in
-//    registers as e~ = se * log2(e)/kT; electrodes hold V * log2(e)/kT.
-//  * With lt = log2(nu * tc) and c = scaled Coulomb term I_0*R/d a rate is
-//        2^(lt + min(0, e~_src - e~_dst + c)) = 2^min(lt, b + e~_src - e~_dst),
-//    b = lt + c: one v_exp_f32 per pair and no FP32 underflow of the 10^-50
-//    prefactors.  Where the pair terms come from is a policy:
-//      TabB  {lt, b} pair table + c table in LDS, shared by the W waves of a
-//            workgroup that run W trajectories of one layout (fewest VALU ops);
-//      TabC  {lt, c} pair table in LDS (smaller; largest networks);
-//      Pos   positions only (16 B per site in LDS per wave): d, lt and c are
-//            computed per pair, ~2.5x the VALU work but no table, so waves of
-//            different layouts fill a CU independently.
-//  * Per hop, the smaller of {sources} (occupied acceptors + electrodes) and
-//    {targets} (empty acceptors + electrodes) is walked four members at a time
-//    with a scalar loop while the lanes cover every site of the other role:
-//    row sums R_lane accumulate in registers with no cross-lane traffic.
-//  * Event pick: FP64 wave prefix sum of R (DPP), Philox event uniform * total,
-//    ballot for the first lane whose prefix exceeds it, then one lane-parallel
-//    pass over that lane's pairs (FP32 DPP scan) for the partner site.
-//  * Hop: occupation bits flip in SGPRs; energies get the O(N) rank-1 Coulomb
-//    update of makeJump (simulation.go:107-130); electrode tallies stay in the
-//    electrode lanes' registers; the clock is accumulated in FP64.
-//  * At every multiple of `recompute_interval` hops the energies are rebuilt in
-//    FP64 from E_constant and the I_0*R/d table, cancelling FP32 drift of the
-//    updates (the Go reference never re-synchronises).
-//  * Uniforms come from one Philox block per lane every 64 hops (lane k holds
-//    hop h0+k), so the RNG costs two readlanes per hop.
-//  * A trajectory runs the absolute hop range [h_begin, h_end) of its budget
-//    and, with `resume`, starts from the state its previous launch wrote.
-//    Ranges that start at multiples of the recompute interval continue the
-//    trajectory exactly as one long launch would (same energies, same stream).
-//
-// Workgroups (table policies) or waves (Pos) pull work from a global atomic
-// queue until it is drained, so one launch serves uneven batches.
-#include "kmc_device.h"
+// Workgroups (table policies) or waves (positions) pull work from a global
+// atomic queue until it is drained, so one launch serves uneven batches.  The
+// per-trajectory hop loop is run_trajectory (kmc_traj.h).
+#include "kmc_traj.h"
 
 namespace kmc {
-
-template <int C>
-__device__ __forceinline__ void recompute_energies(float (&se)[C], const uint64_t (&occ)[OCCW],
-                                                   const LayoutDev& L, const EsetDev& es,
-                                                   int lane) {
-  const int N = L.N, S = L.S;
-  const double sc = (double)L.scale2;
-#pragma unroll
-  for (int c = 0; c < C; ++c) {
-    const int s = lane + 64 * c;
-    float v = 0.f;
-    if (s < N) {
-      double acc = 0.0;
-      for (int j = 0; j < N; ++j)
-        if (!((occ[j >> 6] >> (j & 63)) & 1ull) && j != s) acc += L.coul64[j * N + s];
-      v = (float)((es.E[s] - acc) * sc);
-    } else if (s < S) {
-      v = (float)(es.V[s - N] * sc);
-    }
-    se[c] = v;
-  }
-}
-
-// ---------------------------------------------------------------------------
-// Pair-term policies.  For a uniform member u of the walked set (u < 0: none)
-// and the lane sites s = lane + 64c, pair(pre(u), c, diff) is log2 of the rate
-// of the pair (src, dst) with e~_src - e~_dst = diff; coul(u, c) is c(s, u) for
-// the rank-1 energy update (u an acceptor).  prefetch(u) issues the loads that
-// pair() needs, so four members can be in flight together.
-// ---------------------------------------------------------------------------
-template <int C, bool BT>
-struct TabPolicy {
-  const float2* tab;   // [S*S + 192]: entry (s, u) at u*S + s; column S is all -inf
-  const float* ctab;   // BT: [N*S + 192] c at u*S + s
-  int S, lane;
-  struct Pre {
-    float2 e[C];
-  };
-  __device__ __forceinline__ Pre prefetch(int u) const {
-    Pre p;
-    const float2* col = tab + (u < 0 ? S * S : u * S) + lane;
-#pragma unroll
-    for (int c = 0; c < C; ++c) p.e[c] = col[64 * c];
-    return p;
-  }
-  __device__ __forceinline__ float pair(const Pre& p, int c, float diff) const {
-    if constexpr (BT) return fminf(p.e[c].x, p.e[c].y + diff);
-    else return p.e[c].x + fminf(0.f, diff + p.e[c].y);
-  }
-  __device__ __forceinline__ float coul(int u, int c) const {
-    if constexpr (BT) return ctab[u * S + lane + 64 * c];
-    else return tab[u * S + lane + 64 * c].y;
-  }
-};
-
-template <int C>
-struct PosPolicy {
-  const float4* pos;   // [S] per wave: x, y, z, 1 for acceptors / 0 for electrodes
-  float px[C], py[C], pz[C], kcl[C];   // the lane sites' coordinates and c scale
-  bool lane_el[C];
-  float l0, mkl;       // lt = l0 + mkl * d
-  int lane;
-  struct Pre {
-    float4 q;
-  };
-  __device__ __forceinline__ void init(const float4* p, const LayoutDev& L, int ln) {
-    pos = p;
-    lane = ln;
-    l0 = L.pos_l0;
-    mkl = L.pos_mkl;
-#pragma unroll
-    for (int c = 0; c < C; ++c) {
-      const int s = lane + 64 * c;
-      const float4 q = s < L.S ? pos[s] : make_float4(1e6f, 1e6f, 1e6f, 0.f);
-      px[c] = q.x;
-      py[c] = q.y;
-      pz[c] = q.z;
-      kcl[c] = q.w * L.pos_kc;
-      lane_el[c] = s >= L.N && s < L.S;
-    }
-  }
-  __device__ __forceinline__ Pre prefetch(int u) const {
-    Pre p;
-    p.q = u < 0 ? make_float4(0.f, 0.f, 0.f, -1.f) : pos[u];
-    return p;
-  }
-  __device__ __forceinline__ float dist_rinv(const float4& q, int c, float& rinv) const {
-    const float dx = px[c] - q.x, dy = py[c] - q.y, dz = pz[c] - q.z;
-    const float d2 = fmaxf(dx * dx + dy * dy + dz * dz, 1e-30f);
-    rinv = __builtin_amdgcn_rsqf(d2);
-    return d2 * rinv;
-  }
-  __device__ __forceinline__ float pair(const Pre& p, int c, float diff) const {
-    float rinv;
-    const float d = dist_rinv(p.q, c, rinv);
-    const float lt = fmaf(mkl, d, l0);
-    const float cc = rinv * kcl[c] * p.q.w;
-    const float x = lt + fminf(0.f, diff + cc);
-    // padding members and electrode-electrode pairs (an electrode with itself
-    // included) have no rate
-    return (p.q.w < 0.f || (p.q.w == 0.f && lane_el[c])) ? -INFINITY : x;
-  }
-  __device__ __forceinline__ float coul(int u, int c) const {
-    const float4 q = pos[u];
-    float rinv;
-    (void)dist_rinv(q, c, rinv);
-    return (u == lane + 64 * c) ? 0.f : rinv * kcl[c] * q.w;
-  }
-};
-
-// Row sums over the uniform set U (bits of umask), lanes covering the other role.
-// UDST: U = targets, lanes = sources; otherwise U = sources, lanes = targets.
-template <int C, bool UDST, class Pol>
-__device__ __forceinline__ void row_sums(const Pol& pol, const uint64_t (&umask)[C],
-                                         const float (&v)[C], const float (&se)[C],
-                                         float (&R)[C]) {
-  constexpr int U = 4;
-#pragma unroll
-  for (int cu = 0; cu < C; ++cu) {
-    uint64_t m = umask[cu];
-    while (m) {
-      int b[U], uu[U];
-#pragma unroll
-      for (int k = 0; k < U; ++k) {
-        if (m) {
-          b[k] = __builtin_ctzll(m);
-          m &= m - 1;
-          uu[k] = 64 * cu + b[k];
-        } else {
-          b[k] = 0;
-          uu[k] = -1;
-        }
-      }
-      typename Pol::Pre pre[U];
-#pragma unroll
-      for (int k = 0; k < U; ++k) pre[k] = pol.prefetch(uu[k]);
-#pragma unroll
-      for (int k = 0; k < U; ++k) {
-        const float su = readlane_f32(se[cu], b[k]);
-#pragma unroll
-        for (int c = 0; c < C; ++c) {
-          const float diff = UDST ? (v[c] - su) : (su - v[c]);
-          R[c] += fast_exp2(pol.pair(pre[k], c, diff));
-        }
-      }
-    }
-  }
-}
-
-template <int C>
-__device__ __forceinline__ float pick_lane_value(const float (&x)[C], int site) {
-  const int cs = site >> 6, ls = site & 63;
-  float r = readlane_f32(x[0], ls);
-#pragma unroll
-  for (int c = 1; c < C; ++c)
-    if (cs == c) r = readlane_f32(x[c], ls);
-  return r;
-}
-
-template <int C, bool REC, class Pol>
-__device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, const Pol& pol,
-                               int lane) {
-  const TrajDev td = p.trajs[t];
-  const EsetDev es = p.esets[td.eset];
-  const int N = L.N, S = L.S;
-  TrajOut* o = p.out + t;
-  const int64_t total_hops = td.prehops + td.hops;
-  const int64_t h_begin = p.resume ? td.h_begin : 0;
-  const int64_t h_end = p.resume ? (td.h_end < total_hops ? td.h_end : total_hops) : total_hops;
-  const bool cont = p.resume && h_begin > 0;
-
-  uint64_t occ[OCCW];
-  float se[C];
-  int eo[C];
-  double ot[C];
-#pragma unroll
-  for (int w = 0; w < OCCW; ++w) occ[w] = cont ? o->occ[w] : td.occ[w];
-#pragma unroll
-  for (int c = 0; c < C; ++c) {
-    const int s = lane + 64 * c;
-    eo[c] = (cont && s >= N && s < S) ? (int)o->eo[s - N] : 0;
-    ot[c] = (cont && REC && p.occ_time && s < N) ? p.occ_time[(size_t)t * NMAX + s] : 0.0;
-  }
-  double time = cont ? o->time : 0.0;
-  int32_t status = cont ? o->status : (int32_t)ST_OK;
-  recompute_energies<C>(se, occ, L, es, lane);
-
-  const int K = p.recompute_interval;
-  float lg_lane = 0.f;    // log2(1 - u_clock) of hop (h & ~63) + lane
-  double ue_lane = 0.0;   // event uniform of that hop
-  int64_t h = h_begin;
-  for (; h < h_end && status == ST_OK; ++h) {
-    if (h == h_begin || (h & 63) == 0) {
-      double uc;
-      stream_uniforms(td.seed, td.hop0 + (uint64_t)((h & ~(int64_t)63) + lane), uc, ue_lane);
-      lg_lane = fast_log2((float)(1.0 - uc));
-    }
-    if (h == td.prehops) {
-      time = 0.0;
-#pragma unroll
-      for (int c = 0; c < C; ++c) {
-        eo[c] = 0;
-        ot[c] = 0.0;
-      }
-    }
-    if (h != h_begin && h % K == 0) recompute_energies<C>(se, occ, L, es, lane);
-
-    // roles of the lanes' sites
-    uint64_t srcm[C], dstm[C];
-    bool occd[C];
-    int nsrc = 0, ndst = 0;
-#pragma unroll
-    for (int c = 0; c < C; ++c) {
-      const int s = lane + 64 * c;
-      const bool isacc = s < N;
-      const bool isel = s >= N && s < S;
-      occd[c] = isacc && ((occ[s >> 6] >> (s & 63)) & 1ull);
-      srcm[c] = ballot(isel || occd[c]);
-      dstm[c] = ballot(isel || (isacc && !occd[c]));
-      nsrc += __builtin_popcountll(srcm[c]);
-      ndst += __builtin_popcountll(dstm[c]);
-    }
-    const bool udst = ndst <= nsrc;   // walk the smaller set
-    float v[C], R[C];
-#pragma unroll
-    for (int c = 0; c < C; ++c) {
-      R[c] = 0.f;
-      const bool lane_ok = udst ? ((srcm[c] >> lane) & 1ull) : ((dstm[c] >> lane) & 1ull);
-      v[c] = lane_ok ? se[c] : (udst ? -INFINITY : INFINITY);
-    }
-    if (udst) row_sums<C, true>(pol, dstm, v, se, R);
-    else row_sums<C, false>(pol, srcm, v, se, R);
-
-    // FP64 prefix over lane rows
-    double incl[C];
-    double total = 0.0;
-#pragma unroll
-    for (int c = 0; c < C; ++c) {
-      incl[c] = wave_scan_f64((double)R[c]) + total;
-      total = readlane_f64(incl[c], 63);
-    }
-    if (!(total > 0.0) || !(total < INFINITY)) {
-      status = ST_ZERO_RATE;
-      break;
-    }
-    const float lg = readlane_f32(lg_lane, (int)(h & 63));
-    const double ue = readlane_f64(ue_lane, (int)(h & 63));
-    // exponential waiting time -ln(1-u)/total, FP32 quotient into the FP64 clock
-    const double dt = (double)(lg * -0.69314718055994531f * __builtin_amdgcn_rcpf((float)total));
-    const double target = ue * total;
-
-    int sel = -1;
-    double excl = 0.0;
-#pragma unroll
-    for (int c = 0; c < C; ++c) {
-      const uint64_t m = ballot(incl[c] > target);
-      if (sel < 0 && m) {
-        const int l = __builtin_ctzll(m);
-        sel = 64 * c + l;
-        excl = readlane_f64(incl[c] - (double)R[c], l);
-      }
-    }
-    if (sel < 0) {   // target rounded onto the total: last lane with a nonzero row
-#pragma unroll
-      for (int c = 0; c < C; ++c) {
-        const uint64_t m = ballot(R[c] > 0.f);
-        if (m) {
-          const int l = 63 - __builtin_clzll(m);
-          sel = 64 * c + l;
-          excl = readlane_f64(incl[c] - (double)R[c], l);
-        }
-      }
-    }
-    const float rr = (float)(target - excl);
-
-    // partner of `sel`: lane-parallel pass over its row, in site order
-    const float s_sel = pick_lane_value<C>(se, sel);
-    const typename Pol::Pre pre = pol.prefetch(sel);
-    float run = 0.f;
-    int pick = -1, last_nz = -1;
-#pragma unroll
-    for (int c = 0; c < C; ++c) {
-      const bool inU = udst ? ((dstm[c] >> lane) & 1ull) : ((srcm[c] >> lane) & 1ull);
-      const float diff = udst ? (s_sel - se[c]) : (se[c] - s_sel);
-      float r = fast_exp2(pol.pair(pre, c, diff));
-      r = inU ? r : 0.f;
-      const float cum = wave_scan_f32(r) + run;
-      const uint64_t nz = ballot(r > 0.f);
-      const uint64_t hit = ballot(r > 0.f && cum > rr);
-      if (pick < 0) {
-        if (nz) last_nz = 64 * c + 63 - __builtin_clzll(nz);
-        if (hit) pick = 64 * c + __builtin_ctzll(hit);
-      }
-      run = readlane_f32(cum, 63);
-    }
-    if (pick < 0) pick = last_nz;
-    if (pick < 0) {
-      status = ST_ZERO_RATE;
-      break;
-    }
-    const int src = udst ? sel : pick;
-    const int dst = udst ? pick : sel;
-    const bool tally = h >= td.prehops;
-
-    if (REC && tally && !p.post_hop_record) {
-#pragma unroll
-      for (int c = 0; c < C; ++c)
-        if (occd[c]) ot[c] += dt;
-    }
-    if (p.traffic && tally && lane == 0) {   // net traffic, Go convention (simulation.go:310-311)
-      double* tr = p.traffic + (size_t)t * p.traffic_stride;
-      tr[src * S + dst] += 1.0;
-      tr[dst * S + src] -= 1.0;
-    }
-    // perform hop: occupation, rank-1 energy update, tallies
-    if (src < N) {
-      occ[src >> 6] &= ~(1ull << (src & 63));
-#pragma unroll
-      for (int c = 0; c < C; ++c) se[c] -= pol.coul(src, c);
-    } else {
-#pragma unroll
-      for (int c = 0; c < C; ++c)
-        if (lane + 64 * c == src) eo[c] -= 1;
-    }
-    if (dst < N) {
-      occ[dst >> 6] |= 1ull << (dst & 63);
-#pragma unroll
-      for (int c = 0; c < C; ++c) se[c] += pol.coul(dst, c);
-    } else {
-#pragma unroll
-      for (int c = 0; c < C; ++c)
-        if (lane + 64 * c == dst) eo[c] += 1;
-    }
-    if (REC && tally && p.post_hop_record) {
-#pragma unroll
-      for (int c = 0; c < C; ++c) {
-        const int s = lane + 64 * c;
-        if (s < N && ((occ[s >> 6] >> (s & 63)) & 1ull)) ot[c] += dt;
-      }
-    }
-    time += dt;
-  }
-
-  if (lane == 0) {
-    o->time = time;
-    o->hops_done = h > td.prehops ? h - td.prehops : 0;
-    o->status = status;
-#pragma unroll
-    for (int w = 0; w < OCCW; ++w) o->occ[w] = occ[w];
-  }
-#pragma unroll
-  for (int c = 0; c < C; ++c) {
-    const int s = lane + 64 * c;
-    if (s >= N && s < S) o->eo[s - N] = eo[c];
-    if (REC && p.occ_time && s < N) p.occ_time[(size_t)t * NMAX + s] = ot[c];
-  }
-}
 
 // Table policies: the workgroup's W waves run W trajectories of one layout and
 // share its table in LDS: the pair table (S*S + 192 float2) and, for BT, the
@@ -404,7 +13,11 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
 template <int C, int W, bool REC, bool BT>
 __global__ void __launch_bounds__(64 * W) kmc_batch_fp32_kernel(BatchParams p) {
   extern __shared__ __align__(16) unsigned char smem[];
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  // wave index and everything read from the shared group slot are wave-uniform:
+  // readfirstlane lets the compiler keep them (and the trajectory state derived
+  // from them) in scalar registers with scalar branches
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
   float2* tab = reinterpret_cast<float2*>(smem);
   __shared__ int s_group;
   __shared__ int s_layout;
@@ -413,12 +26,12 @@ __global__ void __launch_bounds__(64 * W) kmc_batch_fp32_kernel(BatchParams p) {
     __syncthreads();
     if (threadIdx.x == 0) s_group = atomicAdd(p.queue, 1);
     __syncthreads();
-    const int g = s_group;
+    const int g = __builtin_amdgcn_readfirstlane(s_group);
     if (g >= p.ngroups) break;
     const GroupDev gd = p.groups[g];
     const LayoutDev L = p.layouts[gd.layout];
     float* ctab = reinterpret_cast<float*>(tab + L.S * L.S + 64 * 3);
-    if (gd.layout != s_layout) {
+    if (gd.layout != __builtin_amdgcn_readfirstlane(s_layout)) {
       const int n = L.S * L.S + 64 * 3;
       const float2* src = BT ? L.tabB : L.tab32;
       for (int k = threadIdx.x; k < n; k += 64 * W) tab[k] = src[k];
@@ -474,6 +87,21 @@ KMC_INST(3, 1) KMC_INST(3, 2) KMC_INST(3, 4) KMC_INST(3, 8)
 
 // Zeroes the work-queue head (a kernel node, so a captured graph can replay it).
 __global__ void kmc_queue_reset_kernel(int32_t* q) { *q = 0; }
+
+// Scatters n staged energy sets into their device slots, one workgroup per set:
+// set i (id ids[i], nps[2i] acceptors, nps[2i+1] electrodes) takes E[i*NMAX ...]
+// and V[i*PMAX ...].
+__global__ void kmc_eset_scatter_kernel(const EsetDev* esets, const int32_t* ids, const int32_t* nps,
+                                        const double* E, const double* V, int64_t n) {
+  const int64_t i = blockIdx.x;
+  if (i >= n) return;
+  const EsetDev e = esets[ids[i]];
+  const int N = nps[2 * i], P = nps[2 * i + 1];
+  double* Ed = const_cast<double*>(e.E);
+  double* Vd = const_cast<double*>(e.V);
+  for (int k = threadIdx.x; k < N; k += blockDim.x) Ed[k] = E[i * NMAX + k];
+  for (int k = threadIdx.x; k < P; k += blockDim.x) Vd[k] = V[i * PMAX + k];
+}
 
 size_t batch_lds_bytes(bool bt, int S, int N) {
   size_t b = ((size_t)S * S + 64 * 3) * sizeof(float2);

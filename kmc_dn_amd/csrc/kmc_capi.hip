@@ -27,9 +27,14 @@ __global__ void kmc_batch_fp32_kernel(BatchParams);
 template <int C, int W, bool REC>
 __global__ void kmc_batch_pos_kernel(BatchParams);
 __global__ void kmc_queue_reset_kernel(int32_t*);
+__global__ void kmc_eset_scatter_kernel(const EsetDev*, const int32_t*, const int32_t*, const double*,
+                                        const double*, int64_t);
 size_t plain_lds_bytes(int S, int N, bool stage);
 size_t batch_lds_bytes(bool bt, int S, int N);
 size_t pos_lds_bytes(int C, int W);
+template <int C, int T, bool REC, int FORM>
+__global__ void kmc_team_kernel(BatchParams);
+size_t team_lds_bytes(int form, int C, int T, int S, int N);
 
 namespace {
 
@@ -55,17 +60,24 @@ int fail(int code, const char* fmt, ...) {
 struct DevBuf {
   void* p = nullptr;
   size_t n = 0;
+  size_t cap = 0;
   ~DevBuf() {
     if (p) (void)hipFree(p);
   }
+  // Contents are not preserved; an allocation that already holds `bytes` is reused.
   int alloc(size_t bytes) {
+    if (p && bytes <= cap && bytes > 0) {
+      n = bytes;
+      return KMC_OK;
+    }
     if (p) (void)hipFree(p);
     p = nullptr;
-    n = bytes;
+    n = cap = bytes;
     if (bytes == 0) return KMC_OK;
     hipError_t e = hipMalloc(&p, bytes);
     if (e != hipSuccess) {
       p = nullptr;
+      n = cap = 0;
       return fail(KMC_ERR_NOMEM, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
     }
     return KMC_OK;
@@ -104,10 +116,27 @@ struct Batch {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   hipGraphExec_t graph = nullptr;
   int64_t graph_key = -1;
+  // size-bucket plan: per bucket its groups, largest S / N, and a side stream
+  // with fork/join events so that the bucket kernels run side by side
+  std::vector<kmc_bucket> plan;
+  struct Sub {
+    DevBuf groups;
+    int32_t ngroups = 0;
+    int smax = 0, nmax = 0;
+  };
+  Sub subs[KMC_MAX_BUCKETS];
+  bool subs_built = false;
+  hipStream_t side[KMC_MAX_BUCKETS] = {};
+  hipEvent_t fork = nullptr, join[KMC_MAX_BUCKETS] = {};
   ~Batch() {
     if (graph) (void)hipGraphExecDestroy(graph);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
+    if (fork) (void)hipEventDestroy(fork);
+    for (int b = 0; b < KMC_MAX_BUCKETS; ++b) {
+      if (join[b]) (void)hipEventDestroy(join[b]);
+      if (side[b]) (void)hipStreamDestroy(side[b]);
+    }
   }
 };
 
@@ -123,6 +152,7 @@ struct Engine {
   std::vector<EsetDev> esets_host;
   std::vector<int> eset_layout;
   DevBuf layouts_dev, esets_dev;
+  DevBuf stage;   // staged energy sets of kmc_eset_write_many
   std::vector<Batch*> batches;
   bool layouts_dirty = true, esets_dirty = true;
 };
@@ -401,6 +431,63 @@ int eset_destroy_locked(int32_t id) {
   return KMC_OK;
 }
 
+int eset_write_locked(int32_t id, const double* Ec, const double* V) {
+  Engine& E = eng();
+  if (id < 0 || id >= (int)E.eset_layout.size() || E.eset_layout[id] < 0)
+    return fail(KMC_ERR_ARGS, "bad energy-set id %d", id);
+  const Layout* L = E.layouts[E.eset_layout[id]];
+  if ((L->N && !Ec) || (L->P && !V)) return fail(KMC_ERR_ARGS, "null E_constant or V");
+  for (int i = 0; i < L->N; ++i)
+    if (!std::isfinite(Ec[i])) return fail(KMC_ERR_ARGS, "E_constant[%d] is not finite", i);
+  for (int i = 0; i < L->P; ++i)
+    if (!std::isfinite(V[i])) return fail(KMC_ERR_ARGS, "V[%d] is not finite", i);
+  const EsetDev& e = E.esets_host[id];
+  if (L->N) HIPOK(hipMemcpy(const_cast<double*>(e.E), Ec, sizeof(double) * L->N, hipMemcpyHostToDevice));
+  if (L->P) HIPOK(hipMemcpy(const_cast<double*>(e.V), V, sizeof(double) * L->P, hipMemcpyHostToDevice));
+  return KMC_OK;
+}
+
+int eset_write_many_locked(int64_t n, const int32_t* ids, const double* Ec, const double* V) {
+  Engine& E = eng();
+  if (n == 0) return KMC_OK;
+  if (n < 0 || n > (1 << 24) || !ids || !Ec || !V) return fail(KMC_ERR_ARGS, "bad energy-set list");
+  std::vector<int32_t> head((size_t)3 * n);   // ids, then (N, P) pairs
+  for (int64_t i = 0; i < n; ++i) {
+    const int32_t id = ids[i];
+    if (id < 0 || id >= (int)E.eset_layout.size() || E.eset_layout[id] < 0)
+      return fail(KMC_ERR_ARGS, "entry %lld: bad energy-set id %d", (long long)i, id);
+    const Layout* L = E.layouts[E.eset_layout[id]];
+    for (int k = 0; k < L->N; ++k)
+      if (!std::isfinite(Ec[i * NMAX + k]))
+        return fail(KMC_ERR_ARGS, "entry %lld: E_constant[%d] is not finite", (long long)i, k);
+    for (int k = 0; k < L->P; ++k)
+      if (!std::isfinite(V[i * PMAX + k]))
+        return fail(KMC_ERR_ARGS, "entry %lld: V[%d] is not finite", (long long)i, k);
+    head[i] = id;
+    head[n + 2 * i] = L->N;
+    head[n + 2 * i + 1] = L->P;
+  }
+  int st = sync_registry();
+  if (st) return st;
+  const size_t hb = ((sizeof(int32_t) * 3 * n + 255) / 256) * 256;
+  const size_t eb = sizeof(double) * NMAX * n, vb = sizeof(double) * PMAX * n;
+  HIPOK(hipStreamSynchronize(E.stream));   // the previous scatter has read the staging buffer
+  if ((st = E.stage.alloc(hb + eb + vb))) return st;
+  char* d = E.stage.as<char>();
+  HIPOK(hipMemcpy(d, head.data(), sizeof(int32_t) * 3 * n, hipMemcpyHostToDevice));
+  HIPOK(hipMemcpy(d + hb, Ec, eb, hipMemcpyHostToDevice));
+  HIPOK(hipMemcpy(d + hb + eb, V, vb, hipMemcpyHostToDevice));
+  const EsetDev* es = E.esets_dev.as<EsetDev>();
+  const int32_t* di = reinterpret_cast<const int32_t*>(d);
+  const int32_t* dn = di + n;
+  const double* dE = reinterpret_cast<const double*>(d + hb);
+  const double* dV = reinterpret_cast<const double*>(d + hb + eb);
+  void* args[] = {&es, &di, &dn, &dE, &dV, &n};
+  HIPOK(hipLaunchKernel((const void*)kmc_eset_scatter_kernel, dim3((unsigned)n), dim3(64), args, 0,
+                        E.stream));
+  return KMC_OK;
+}
+
 Batch* get_batch(int32_t id) {
   Engine& E = eng();
   if (id < 0 || id >= (int)E.batches.size()) return nullptr;
@@ -454,6 +541,7 @@ int batch_prepare(Batch* B, const kmc_traj* t, int64_t ntraj) {
     B->nmax = std::max(B->nmax, E.layouts[x.layout]->N);
   }
   B->group_mode = -1;
+  B->subs_built = false;
   if (B->graph) {
     (void)hipGraphExecDestroy(B->graph);
     B->graph = nullptr;
@@ -472,7 +560,7 @@ int batch_create_locked(const kmc_traj* t, int64_t ntraj, int32_t* out) {
   }
   B->capacity = ntraj;
   if ((st = upload(B->trajs, B->host)) || (st = B->out.alloc(sizeof(TrajOut) * ntraj)) ||
-      (st = B->queue.alloc(64))) {
+      (st = B->queue.alloc(64 * KMC_MAX_BUCKETS))) {
     delete B;
     return st;
   }
@@ -568,6 +656,12 @@ int build_groups(Batch* B, int W) {
   return upload(B->groups, g);
 }
 
+int bucket_of(const Batch* B, int S, int64_t budget) {
+  for (size_t b = 0; b < B->plan.size(); ++b)
+    if (S <= B->plan[b].s_max && budget >= B->plan[b].min_hops) return (int)b;
+  return -1;
+}
+
 using BatchKernel = void (*)(BatchParams);
 
 BatchKernel pick_batch_kernel(int C, int W, bool rec, bool bt) {
@@ -581,6 +675,19 @@ BatchKernel pick_batch_kernel(int C, int W, bool rec, bool bt) {
   KMC_PICK(1, 1) KMC_PICK(1, 2) KMC_PICK(1, 4) KMC_PICK(1, 8)
   KMC_PICK(2, 1) KMC_PICK(2, 2) KMC_PICK(2, 4) KMC_PICK(2, 8)
   KMC_PICK(3, 1) KMC_PICK(3, 2) KMC_PICK(3, 4) KMC_PICK(3, 8)
+#undef KMC_PICK
+  return nullptr;
+}
+
+BatchKernel pick_team_kernel(int C, int T, bool rec, int form) {
+#define KMC_PICK(CC, TT, FF)                                                         \
+  if (C == CC && T == TT && form == FF)                                              \
+    return rec ? (BatchKernel)kmc_team_kernel<CC, TT, true, FF>                      \
+               : (BatchKernel)kmc_team_kernel<CC, TT, false, FF>;
+  KMC_PICK(1, 2, 1) KMC_PICK(1, 2, 2) KMC_PICK(1, 2, 3) KMC_PICK(1, 4, 1) KMC_PICK(1, 4, 2)
+  KMC_PICK(1, 4, 3) KMC_PICK(2, 2, 1) KMC_PICK(2, 2, 2) KMC_PICK(2, 2, 3) KMC_PICK(2, 4, 1)
+  KMC_PICK(2, 4, 2) KMC_PICK(2, 4, 3) KMC_PICK(3, 2, 1) KMC_PICK(3, 2, 2) KMC_PICK(3, 2, 3)
+  KMC_PICK(3, 4, 1) KMC_PICK(3, 4, 2) KMC_PICK(3, 4, 3)
 #undef KMC_PICK
   return nullptr;
 }
@@ -601,79 +708,56 @@ struct Fp32Launch {
   BatchKernel k = nullptr;
   int grid = 0, block = 0;
   size_t lds = 0;
-  size_t traffic_bytes = 0;
   BatchParams p{};
 };
 
-// Everything that allocates or queries happens here, outside any graph capture.
-int prepare_fp32(Batch* B, const kmc_run_opts& o, Fp32Launch& L) {
+// One work-queue kernel over `ngroups` groups.  form: 1 {lt, lt+c} + c table,
+// 2 {lt, c} table, 3 positions; smax/nmax bound the S and N of its layouts.
+int setup_launch(Batch* B, const kmc_run_opts& o, int form, int W, int team, int smax, int nmax,
+                 const DevBuf& groups, int32_t ngroups, int32_t* queue, int grid_cap,
+                 Fp32Launch& L) {
   Engine& E = eng();
-  const int C = (B->smax + 63) / 64;
-  const bool pos = o.table_form == 3;
-  int W = o.waves_per_group;
-  if (W <= 0) W = pos ? 4 : (C == 1 ? 4 : (C == 2 ? 2 : 1));
-  if (W != 1 && W != 2 && W != 4 && W != 8) return fail(KMC_ERR_ARGS, "waves_per_group must be 1, 2, 4 or 8");
+  const int C = (smax + 63) / 64;
   const bool rec = (o.record & 1) != 0;
-  const bool trf = (o.record & 2) != 0;
   const size_t lds_cap = (size_t)160 * 1024;
   BatchKernel k = nullptr;
   size_t lds = 0;
-  if (pos) {
-    for (const TrajDev& t : B->host)
-      if (!E.layouts[t.layout]->dev.has_pos)
-        return fail(KMC_ERR_ARGS, "layout %d has no positions (kmc_layout_set_positions)", t.layout);
+  if (team > 1) {
+    k = pick_team_kernel(C, team, rec, form);
+    lds = team_lds_bytes(form, C, team, smax, nmax);
+    W = team;   // block of 64 * team threads, one trajectory per workgroup
+  } else if (form == 3) {
     k = pick_pos_kernel(C, W, rec);
     lds = pos_lds_bytes(C, W);
   } else {
-    bool bt;
-    if (o.table_form == 1) bt = true;
-    else if (o.table_form == 2) bt = false;
-    else if (o.table_form == 0) bt = batch_lds_bytes(true, B->smax, B->nmax) <= lds_cap;
-    else return fail(KMC_ERR_ARGS, "table_form must be 0, 1, 2 or 3");
-    k = pick_batch_kernel(C, W, rec, bt);
-    lds = batch_lds_bytes(bt, B->smax, B->nmax);
+    k = pick_batch_kernel(C, W, rec, form == 1);
+    lds = batch_lds_bytes(form == 1, smax, nmax);
   }
   if (!k) return fail(KMC_ERR_LIMIT, "no kernel for %d site chunks", C);
   if (lds > lds_cap)
-    return fail(KMC_ERR_LIMIT, "LDS %zu bytes for S=%d exceeds 160 KiB", lds, B->smax);
+    return fail(KMC_ERR_LIMIT, "LDS %zu bytes for S=%d exceeds 160 KiB (table form %d)", lds, smax, form);
   HIPOK(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  int st = build_groups(B, pos ? 1 : W);   // positions: each wave pulls one trajectory
-  if (st) return st;
   int per_cu = 0;
   HIPOK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k, 64 * W, lds));
   if (per_cu <= 0) return fail(KMC_ERR_LIMIT, "kernel does not fit on a CU (LDS %zu)", lds);
   int grid = per_cu * E.cu_count;
-  if (o.grid_limit > 0) grid = std::min(grid, o.grid_limit);
-  const int work_units = pos ? (B->ngroups + W - 1) / W : B->ngroups;
+  if (grid_cap > 0) grid = std::min(grid, grid_cap);
+  const int work_units = (form == 3 && team <= 1) ? (ngroups + W - 1) / W : ngroups;
   grid = std::max(1, std::min(grid, work_units));
-  if (rec && !B->has_occ_time) {
-    if ((st = B->occ_time.alloc(sizeof(double) * NMAX * B->capacity))) return st;
-    HIPOK(hipMemset(B->occ_time.p, 0, sizeof(double) * NMAX * B->capacity));
-    B->has_occ_time = true;
-  }
-  const int64_t tstride = (int64_t)B->smax * B->smax;
-  if (trf) {
-    if (o.use_graph) return fail(KMC_ERR_ARGS, "traffic recording is not available with use_graph");
-    if (!B->has_traffic) {
-      if ((st = B->traffic.alloc(sizeof(double) * tstride * B->capacity))) return st;
-      B->has_traffic = true;
-    }
-  }
   BatchParams& p = L.p;
   p.trajs = B->trajs.as<TrajDev>();
   p.layouts = E.layouts_dev.as<LayoutDev>();
   p.esets = E.esets_dev.as<EsetDev>();
-  p.groups = B->groups.as<GroupDev>();
-  p.ngroups = B->ngroups;
+  p.groups = groups.as<GroupDev>();
+  p.ngroups = ngroups;
   p.recompute_interval = o.recompute_interval > 0 ? o.recompute_interval : 1024;
-  p.queue = B->queue.as<int32_t>();
+  p.queue = queue;
   p.out = B->out.as<TrajOut>();
   p.occ_time = rec ? B->occ_time.as<double>() : nullptr;
-  p.traffic = trf ? B->traffic.as<double>() : nullptr;
-  p.traffic_stride = tstride;
+  p.traffic = (o.record & 2) ? B->traffic.as<double>() : nullptr;
+  p.traffic_stride = (int64_t)B->smax * B->smax;
   p.post_hop_record = o.post_hop_record;
   p.resume = o.resume ? 1 : 0;
-  L.traffic_bytes = (trf && !o.resume) ? sizeof(double) * (size_t)tstride * B->ntraj : 0;
   L.k = k;
   L.grid = grid;
   L.block = 64 * W;
@@ -681,13 +765,157 @@ int prepare_fp32(Batch* B, const kmc_run_opts& o, Fp32Launch& L) {
   return KMC_OK;
 }
 
-int launch_fp32(Fp32Launch& L, hipStream_t s) {
-  if (L.traffic_bytes) HIPOK(hipMemsetAsync(L.p.traffic, 0, L.traffic_bytes, s));
+// Batch-wide record buffers (occupation time, traffic).
+int prepare_record(Batch* B, const kmc_run_opts& o) {
+  if ((o.record & 1) && !B->has_occ_time) {
+    int st = B->occ_time.alloc(sizeof(double) * NMAX * B->capacity);
+    if (st) return st;
+    HIPOK(hipMemset(B->occ_time.p, 0, sizeof(double) * NMAX * B->capacity));
+    B->has_occ_time = true;
+  }
+  if (o.record & 2) {
+    if (o.use_graph) return fail(KMC_ERR_ARGS, "traffic recording is not available with use_graph");
+    if (!B->has_traffic) {
+      int st = B->traffic.alloc(sizeof(double) * (size_t)B->smax * B->smax * B->capacity);
+      if (st) return st;
+      B->has_traffic = true;
+    }
+  }
+  return KMC_OK;
+}
+
+int check_positions(Batch* B, int bucket) {
+  Engine& E = eng();
+  for (const TrajDev& t : B->host)
+    if ((bucket < 0 || bucket_of(B, E.layouts[t.layout]->S, t.prehops + t.hops) == bucket) &&
+        !E.layouts[t.layout]->dev.has_pos)
+      return fail(KMC_ERR_ARGS, "layout %d has no positions (kmc_layout_set_positions)", t.layout);
+  return KMC_OK;
+}
+
+// Per-bucket groups: runs of <= W same-layout trajectories (one per group for
+// the positions form), longest budgets first.
+int build_bucket_groups(Batch* B) {
+  if (B->subs_built) return KMC_OK;
+  Engine& E = eng();
+  const int nb = (int)B->plan.size();
+  std::vector<std::vector<GroupDev>> g(nb);
+  for (int b = 0; b < nb; ++b) B->subs[b].smax = B->subs[b].nmax = 0;
+  auto bucket_at = [&](int64_t k) {
+    const TrajDev& t = B->host[k];
+    return bucket_of(B, E.layouts[t.layout]->S, t.prehops + t.hops);
+  };
+  int64_t i = 0;
+  while (i < B->ntraj) {
+    const Layout* Ly = E.layouts[B->host[i].layout];
+    const int b = bucket_at(i);
+    if (b < 0)
+      return fail(KMC_ERR_ARGS, "trajectory of layout %d (S=%d, budget %lld) matches no bucket",
+                  B->host[i].layout, Ly->S,
+                  (long long)(B->host[i].prehops + B->host[i].hops));
+    const kmc_bucket& pb = B->plan[b];
+    const int W = (pb.table_form == 3 || pb.team > 1) ? 1 : pb.waves_per_group;
+    int64_t j = i;
+    while (j < B->ntraj && j - i < W && B->host[j].layout == B->host[i].layout && bucket_at(j) == b)
+      ++j;
+    GroupDev gd;
+    gd.layout = B->host[i].layout;
+    gd.first = (int32_t)i;
+    gd.count = (int32_t)(j - i);
+    gd.pad = 0;
+    g[b].push_back(gd);
+    B->subs[b].smax = std::max(B->subs[b].smax, Ly->S);
+    B->subs[b].nmax = std::max(B->subs[b].nmax, Ly->N);
+    i = j;
+  }
+  for (int b = 0; b < nb; ++b) {
+    std::stable_sort(g[b].begin(), g[b].end(), [&](const GroupDev& x, const GroupDev& y) {
+      const TrajDev& u = B->host[x.first];
+      const TrajDev& v = B->host[y.first];
+      return u.prehops + u.hops > v.prehops + v.hops;
+    });
+    B->subs[b].ngroups = (int32_t)g[b].size();
+    int st = upload(B->subs[b].groups, g[b]);
+    if (st) return st;
+  }
+  B->subs_built = true;
+  return KMC_OK;
+}
+
+// Everything that allocates or queries happens here, outside any graph capture.
+int prepare_fp32(Batch* B, const kmc_run_opts& o, std::vector<Fp32Launch>& out) {
+  int st = prepare_record(B, o);
+  if (st) return st;
+  int32_t* q = B->queue.as<int32_t>();
+  if (!B->plan.empty()) {
+    if ((st = build_bucket_groups(B))) return st;
+    // largest networks first: they are the slowest per hop
+    for (int b = (int)B->plan.size() - 1; b >= 0; --b) {
+      const Batch::Sub& sb = B->subs[b];
+      if (sb.ngroups == 0) continue;
+      const kmc_bucket& pb = B->plan[b];
+      if (pb.table_form == 3 && (st = check_positions(B, b))) return st;
+      Fp32Launch L;
+      st = setup_launch(B, o, pb.table_form, pb.waves_per_group, pb.team, sb.smax, sb.nmax,
+                        sb.groups, sb.ngroups, q + 16 * b, pb.grid > 0 ? pb.grid : o.grid_limit, L);
+      if (st) return st;
+      out.push_back(L);
+    }
+    return KMC_OK;
+  }
+  const int C = (B->smax + 63) / 64;
+  int form = o.table_form;
+  if (form == 0) form = batch_lds_bytes(true, B->smax, B->nmax) <= (size_t)160 * 1024 ? 1 : 2;
+  if (form < 1 || form > 3) return fail(KMC_ERR_ARGS, "table_form must be 0, 1, 2 or 3");
+  int W = o.waves_per_group;
+  if (W <= 0) W = form == 3 ? 4 : (C == 1 ? 4 : (C == 2 ? 2 : 1));
+  if (W != 1 && W != 2 && W != 4 && W != 8) return fail(KMC_ERR_ARGS, "waves_per_group must be 1, 2, 4 or 8");
+  const int team = o.team <= 1 ? 1 : o.team;
+  if (team != 1 && team != 2 && team != 4) return fail(KMC_ERR_ARGS, "team must be 1, 2 or 4");
+  if (form == 3 && (st = check_positions(B, -1))) return st;
+  // positions and teams: one trajectory per group
+  if ((st = build_groups(B, (form == 3 || team > 1) ? 1 : W))) return st;
+  Fp32Launch L;
+  if ((st = setup_launch(B, o, form, W, team, B->smax, B->nmax, B->groups, B->ngroups, q,
+                         o.grid_limit, L)))
+    return st;
+  out.push_back(L);
+  return KMC_OK;
+}
+
+int launch_one(const Fp32Launch& L, hipStream_t s) {
   int32_t* q = L.p.queue;
   void* qa[] = {&q};
   HIPOK(hipLaunchKernel((const void*)kmc_queue_reset_kernel, dim3(1), dim3(1), qa, 0, s));
-  void* args[] = {&L.p};
+  BatchParams p = L.p;
+  void* args[] = {&p};
   HIPOK(hipLaunchKernel((const void*)L.k, dim3(L.grid), dim3(L.block), args, L.lds, s));
+  return KMC_OK;
+}
+
+int ensure_side_streams(Batch* B, int n) {
+  if (!B->fork) HIPOK(hipEventCreateWithFlags(&B->fork, hipEventDisableTiming));
+  for (int b = 0; b < n; ++b) {
+    if (!B->side[b]) HIPOK(hipStreamCreateWithFlags(&B->side[b], hipStreamNonBlocking));
+    if (!B->join[b]) HIPOK(hipEventCreateWithFlags(&B->join[b], hipEventDisableTiming));
+  }
+  return KMC_OK;
+}
+
+// Launches on stream s; several launches fork onto the batch's side streams
+// and join back into s (parallel branches when captured into a graph).
+int launch_fp32(Batch* B, const kmc_run_opts& o, const std::vector<Fp32Launch>& Ls, hipStream_t s) {
+  if ((o.record & 2) && !o.resume)
+    HIPOK(hipMemsetAsync(B->traffic.p, 0, sizeof(double) * (size_t)B->smax * B->smax * B->ntraj, s));
+  if (Ls.size() == 1) return launch_one(Ls[0], s);
+  HIPOK(hipEventRecord(B->fork, s));
+  for (size_t b = 0; b < Ls.size(); ++b) {
+    HIPOK(hipStreamWaitEvent(B->side[b], B->fork, 0));
+    int st = launch_one(Ls[b], B->side[b]);
+    if (st) return st;
+    HIPOK(hipEventRecord(B->join[b], B->side[b]));
+  }
+  for (size_t b = 0; b < Ls.size(); ++b) HIPOK(hipStreamWaitEvent(s, B->join[b], 0));
   return KMC_OK;
 }
 
@@ -744,13 +972,15 @@ int batch_run_locked(int32_t id, const kmc_run_opts* opts) {
     return KMC_OK;
   }
   if (o.precision != KMC_FP32_BATCHED) return fail(KMC_ERR_ARGS, "unknown precision %d", o.precision);
-  Fp32Launch L;
-  if ((st = prepare_fp32(B, o, L))) return st;
+  std::vector<Fp32Launch> Ls;
+  if ((st = prepare_fp32(B, o, Ls))) return st;
+  if (Ls.size() > 1 && (st = ensure_side_streams(B, (int)Ls.size()))) return st;
   if (o.use_graph) {
     const int64_t key = (int64_t)(o.waves_per_group & 0xFF) | ((int64_t)(o.record & 1) << 8) |
                         ((int64_t)(o.post_hop_record & 1) << 9) |
                         ((int64_t)(o.recompute_interval & 0xFFFFF) << 10) |
-                        ((int64_t)(o.table_form & 3) << 30) | ((int64_t)(o.resume & 1) << 32);
+                        ((int64_t)(o.table_form & 3) << 30) | ((int64_t)(o.resume & 1) << 32) |
+                        ((int64_t)(o.team & 7) << 33);
     if (!B->graph || B->graph_key != key || o.grid_limit) {
       if (B->graph) {
         (void)hipGraphExecDestroy(B->graph);
@@ -761,7 +991,7 @@ int batch_run_locked(int32_t id, const kmc_run_opts* opts) {
       HIPOK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
       hipGraph_t g;
       HIPOK(hipStreamBeginCapture(cs, hipStreamCaptureModeRelaxed));
-      st = launch_fp32(L, cs);
+      st = launch_fp32(B, o, Ls, cs);
       hipError_t ce = hipStreamEndCapture(cs, &g);
       (void)hipStreamDestroy(cs);
       if (st) return st;
@@ -776,7 +1006,7 @@ int batch_run_locked(int32_t id, const kmc_run_opts* opts) {
     HIPOK(hipEventRecord(B->ev1, s));
   } else {
     HIPOK(hipEventRecord(B->ev0, s));
-    st = launch_fp32(L, s);
+    st = launch_fp32(B, o, Ls, s);
     if (st) return st;
     HIPOK(hipEventRecord(B->ev1, s));
   }
@@ -888,6 +1118,51 @@ int kmc_eset_create(int32_t layout, int64_t count, const double* E_constant, con
   int st = ensure_init();
   if (st) return st;
   return eset_create_locked(layout, count, E_constant, V, first_out);
+}
+
+int kmc_eset_write(int32_t eset, const double* E_constant, const double* V) {
+  Engine& E = eng();
+  std::lock_guard<std::mutex> lk(E.mu);
+  return eset_write_locked(eset, E_constant, V);
+}
+
+int kmc_eset_write_many(int64_t n, const int32_t* ids, const double* E_constant, const double* V) {
+  Engine& E = eng();
+  std::lock_guard<std::mutex> lk(E.mu);
+  int st = ensure_init();
+  if (st) return st;
+  return eset_write_many_locked(n, ids, E_constant, V);
+}
+
+int kmc_batch_set_buckets(int32_t batch, const kmc_bucket* buckets, int32_t nbuckets) {
+  Engine& E = eng();
+  std::lock_guard<std::mutex> lk(E.mu);
+  Batch* B = get_batch(batch);
+  if (!B) return fail(KMC_ERR_ARGS, "bad batch id %d", batch);
+  if (nbuckets < 0 || nbuckets > KMC_MAX_BUCKETS || (nbuckets > 0 && !buckets))
+    return fail(KMC_ERR_ARGS, "nbuckets must be 0..%d", KMC_MAX_BUCKETS);
+  for (int b = 0; b < nbuckets; ++b) {
+    const kmc_bucket& x = buckets[b];
+    if (x.s_max <= 0 || x.s_max > SMAX_FP32)
+      return fail(KMC_ERR_ARGS, "bucket %d: s_max must be within 1..%d", b, SMAX_FP32);
+    if (x.team != 0 && x.team != 1 && x.team != 2 && x.team != 4)
+      return fail(KMC_ERR_ARGS, "bucket %d: team must be 1, 2 or 4", b);
+    if (x.min_hops < 0) return fail(KMC_ERR_ARGS, "bucket %d: negative min_hops", b);
+    if (x.table_form < 1 || x.table_form > 3)
+      return fail(KMC_ERR_ARGS, "bucket %d: table_form must be 1, 2 or 3", b);
+    if (x.waves_per_group != 1 && x.waves_per_group != 2 && x.waves_per_group != 4 &&
+        x.waves_per_group != 8)
+      return fail(KMC_ERR_ARGS, "bucket %d: waves_per_group must be 1, 2, 4 or 8", b);
+    if (x.grid < 0) return fail(KMC_ERR_ARGS, "bucket %d: negative grid", b);
+  }
+  (void)hipStreamSynchronize(E.stream);
+  B->plan.assign(buckets, buckets + nbuckets);
+  B->subs_built = false;
+  if (B->graph) {
+    (void)hipGraphExecDestroy(B->graph);
+    B->graph = nullptr;
+  }
+  return KMC_OK;
 }
 
 int kmc_batch_create(const kmc_traj* trajs, int64_t ntraj, int32_t* batch_out) {

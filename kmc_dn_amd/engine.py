@@ -90,6 +90,22 @@ class Layout:
         L.check(L.load().kmc_layout_set_positions(self.id, float(ab), L.ptr(xyz, L._dp)))
         return self
 
+    def write_energy_set(self, eset: int, E_constant, V) -> None:
+        """Overwrite energy set `eset` of this layout in place (no allocation)."""
+        E = np.ascontiguousarray(E_constant, dtype=np.float64).reshape(self.N)
+        Vv = np.ascontiguousarray(V, dtype=np.float64).reshape(self.P)
+        L.check(L.load().kmc_eset_write(int(eset), L.ptr(E, L._dp), L.ptr(Vv, L._dp)))
+
+    @staticmethod
+    def write_energy_sets(ids, E_pad, V_pad) -> None:
+        """Overwrite energy sets ids[i] from rows of E_pad (n, NMAX) and V_pad (n, PMAX)
+        (each set reads its layout's N and P leading entries) in one upload."""
+        ids = np.ascontiguousarray(ids, dtype=np.int32)
+        E = np.ascontiguousarray(E_pad, dtype=np.float64).reshape(ids.size, L.NMAX)
+        V = np.ascontiguousarray(V_pad, dtype=np.float64).reshape(ids.size, L.PMAX)
+        L.check(L.load().kmc_eset_write_many(ids.size, L.ptr(ids, L._i32p), L.ptr(E, L._dp),
+                                             L.ptr(V, L._dp)))
+
     @staticmethod
     def drop_energy_sets(ids) -> None:
         for i in np.atleast_1d(ids):
@@ -144,13 +160,24 @@ class Batch:
     def run(self, precision: int = L.FP32_BATCHED, waves_per_group: int = 0,
             recompute_interval: int = 0, record: bool = False, post_hop_record: bool = False,
             grid_limit: int = 0, use_graph: bool = False, traffic: bool = False,
-            table_form: int = 0, resume: bool = False) -> None:
+            table_form: int = 0, resume: bool = False, team: int = 1) -> None:
         rec = (1 if record else 0) | (2 if traffic else 0)
         o = L.kmc_run_opts(int(precision), int(waves_per_group), int(recompute_interval),
                            rec, int(post_hop_record), int(grid_limit), int(use_graph),
-                           int(table_form), int(bool(resume)), 0)
+                           int(table_form), int(bool(resume)), int(team))
         L.check(L.load().kmc_batch_run(self.id, C.byref(o)))
         self.recorded = bool(record)
+
+    def set_buckets(self, buckets) -> None:
+        """Bucket plan: entries (s_max, table_form, waves_per_group[, team[, min_hops[,
+        grid]]]); a trajectory goes to the first entry with S <= s_max and budget >=
+        min_hops.  Each bucket runs in its own kernel, all in one fork/join.  [] clears."""
+        arr = (L.kmc_bucket * max(1, len(buckets)))()
+        for i, b in enumerate(buckets):
+            b = [int(x) for x in b] + [0] * (6 - len(b))
+            s_max, form, w, team, min_hops, grid = b[:6]
+            arr[i] = L.kmc_bucket(s_max, form, w, max(1, team), min_hops, grid, 0)
+        L.check(L.load().kmc_batch_set_buckets(self.id, arr, len(buckets)))
 
     def update(self, trajs: np.ndarray) -> None:
         """Replace the descriptors (reusing device buffers when they fit)."""

@@ -26,6 +26,8 @@ kmc_dopant_networks_utils.py:655-676.
 """
 from __future__ import annotations
 
+import functools
+
 import numpy as np
 import scipy.sparse as sp
 import scipy.sparse.linalg as spla
@@ -130,6 +132,12 @@ class Laplace2D:
         return self.solve(bv)
 
 
+@functools.lru_cache(maxsize=8)
+def laplace2d(xdim: float, ydim: float, res: float) -> Laplace2D:
+    """Shared factorisation per mesh (read-only after construction)."""
+    return Laplace2D(float(xdim), float(ydim), float(res))
+
+
 class Interval1D:
     """IntervalMesh(int(xdim//res), 0, xdim) with P1 (exact piecewise-linear solve)."""
 
@@ -167,7 +175,7 @@ def eV_constant(acceptors, electrodes, xdim, ydim, zdim=0.0, res=None, mu=0.0,
         return m.evaluate(m.solve_electrodes(electrodes, static_electrodes, mu), acc[:, 0])
     if zdim == 0:
         r = min(xdim, ydim) / 100 if res is None else res
-        lap = Laplace2D(xdim, ydim, r)
+        lap = laplace2d(xdim, ydim, r)
         V = lap.solve_electrodes(electrodes, static_electrodes, mu)
         return lap.mesh.evaluate(V, acc[:, 0], acc[:, 1])
     return potential_3d_slab(acc, electrodes, xdim, ydim, res, mu, static_electrodes)
@@ -178,7 +186,7 @@ def potential_3d_slab(acceptors, electrodes, xdim, ydim, res=None, mu=0.0,
     """Engine-defined 3D potential: 2D solve of the footprint, constant in z."""
     acc = np.asarray(acceptors, dtype=np.float64).reshape(-1, 3)
     r = min(xdim, ydim) / 100 if res is None else res
-    lap = Laplace2D(xdim, ydim, r)
+    lap = laplace2d(xdim, ydim, r)
     V = lap.solve_electrodes(electrodes, static_electrodes, mu)
     return lap.mesh.evaluate(V, acc[:, 0], acc[:, 1])
 
@@ -191,7 +199,7 @@ class LaplaceBasis:
         acc = np.asarray(acceptors, dtype=np.float64).reshape(-1, 3)
         el = np.asarray(electrode_xy, dtype=np.float64).reshape(-1, 4).copy()
         r = min(xdim, ydim) / 100 if res is None else res
-        self.lap = Laplace2D(xdim, ydim, r)
+        self.lap = laplace2d(xdim, ydim, r)
         P = el.shape[0]
         self.static = np.zeros((0, 4)) if static_electrodes is None else static_electrodes
         cols = []

@@ -190,3 +190,77 @@ def test_resume_in_chunks_is_bit_exact(gpu_engine):
         assert np.array_equal(r.occ_bits, ref.occ_bits)
         assert np.array_equal(r.occ_time, ref.occ_time)
         assert (r.hops_done == hops).all()
+
+
+def test_team_kernels_deterministic_and_resumable(gpu_engine):
+    """A team of 2 or 4 waves per trajectory: repeated and graph runs agree bit for
+    bit, a chunked resume run equals the unchunked run, and the currents agree with
+    the solo kernel statistically (the row sums are added in another order)."""
+    net = random_network(4, 70, 10, I_0=50.0, a=0.1, vmax=1.5)
+    N, P = 70, 10
+    lay = engine.Layout(N, P, net["nu"], net["kT"], net["I_0"], net["R"], net["distances"],
+                        net["transitions_constant"])
+    lay.set_positions(0.1 * net["R"], _positions(net))
+    es = lay.energy_sets(net["E_constant"], net["site_energies"][N:])
+    K, hops = 96, 6144
+    trajs = engine.make_trajs(K, lay.id, es[0], np.arange(K, dtype=np.uint64), hops,
+                              net["occupation"])
+    b = engine.Batch(trajs)
+    cur = {}
+    for form in (1, 2, 3):
+        for team in (1, 2, 4):
+            print(f"[gpu] team={team} form={form}", flush=True)
+            b.run(precision=_lib.FP32_BATCHED, table_form=form, team=team)
+            b.sync()
+            r = b.results()
+            assert (r.status == 0).all() and (r.hops_done == hops).all()
+            if team > 1:
+                b.run(precision=_lib.FP32_BATCHED, table_form=form, team=team, use_graph=True)
+                b.sync()
+                g = b.results()
+                assert np.array_equal(g.eo, r.eo) and np.array_equal(g.time, r.time)
+                # the same trajectories in three resumed chunks
+                for lo, hi in ((0, 2048), (2048, 4096), (4096, hops)):
+                    b.set_state(h_begin=np.full(K, lo), h_end=np.full(K, hi))
+                    b.run(precision=_lib.FP32_BATCHED, table_form=form, team=team, resume=True)
+                    b.sync()
+                c = b.results()
+                assert np.array_equal(c.eo, r.eo) and np.array_equal(c.time, r.time)
+                assert np.array_equal(c.occ_bits, r.occ_bits)
+                b.set_state(h_begin=np.zeros(K), h_end=np.full(K, hops))
+            cur[(form, team)] = r.currents(P)
+    b.close()
+    lay.close()
+    base = cur[(3, 1)]
+    for key, c in cur.items():
+        se = np.sqrt(base.var(0, ddof=1) / K + c.var(0, ddof=1) / K)
+        ok = se > 0
+        z = np.abs(base.mean(0) - c.mean(0))[ok] / se[ok]
+        print(f"[gpu] form/team {key} vs positions solo: max |z| {z.max():.2f}", flush=True)
+        assert z.max() < 4.5
+
+
+def test_budget_buckets_route_long_trajectories_to_teams(gpu_engine):
+    net = random_network(6, 40, 8, I_0=50.0, a=0.1, vmax=1.0)
+    N, P = 40, 8
+    lay = engine.Layout(N, P, net["nu"], net["kT"], net["I_0"], net["R"], net["distances"],
+                        net["transitions_constant"])
+    es = lay.energy_sets(net["E_constant"], net["site_energies"][N:])
+    K = 64
+    hops = np.where(np.arange(K) % 4 == 0, 8192, 2048)
+    trajs = engine.make_trajs(K, lay.id, es[0], np.arange(K, dtype=np.uint64), hops,
+                              net["occupation"])
+    b = engine.Batch(trajs)
+    print("[gpu] budget buckets: long -> team 4, short -> solo", flush=True)
+    b.set_buckets([(64, 1, 1, 4, 8192), (64, 1, 4, 1, 0)])
+    b.run(precision=_lib.FP32_BATCHED, use_graph=True)
+    b.sync()
+    mixed = b.results()
+    b.set_buckets([])
+    for team, sel in ((4, hops == 8192), (1, hops == 2048)):
+        b.run(precision=_lib.FP32_BATCHED, table_form=1, team=team)
+        b.sync()
+        r = b.results()
+        assert np.array_equal(r.eo[sel], mixed.eo[sel]) and np.array_equal(r.time[sel], mixed.time[sel])
+    b.close()
+    lay.close()

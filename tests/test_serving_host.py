@@ -1,0 +1,71 @@
+"""Host logic of the serving loop and the autotuner (no GPU)."""
+import types
+
+import numpy as np
+import pytest
+
+from kmc_dn_amd import _lib, autotune, serving
+
+
+def test_popcount_rows():
+    bits = np.array([[0, 0, 0, 0], [1, 3, 0, 2**63], [2**64 - 1, 0, 0, 0]], dtype=np.uint64)
+    assert list(serving.popcount_rows(bits)) == [0, 4, 64]
+
+
+def test_bytes_per_trajectory_counts_device_records():
+    # descriptor + output record + group entry + E_constant and V doubles
+    assert serving.bytes_per_trajectory(30, 8) == 88 + 184 + 16 + 8 * 38
+
+
+def test_trajectory_seed_is_unique_per_request_and_slot():
+    seeds = {serving.trajectory_seed(r, j) for r in range(50) for j in range(300)}
+    assert len(seeds) == 50 * 300
+
+
+def _stub(N, P, occ_counts, chunk=8192, K=1024):
+    n = len(occ_counts)
+    s = types.SimpleNamespace(chunk=chunk, K=K, REF_COST=serving.Server.REF_COST)
+    s.lN, s.lP = np.array([N]), np.array([P])
+    s.lC = (s.lN + s.lP + 63) // 64
+    s.s_lay = np.zeros(n, np.int64)
+    occ = np.zeros((n, _lib.OCCW), np.uint64)
+    for i, c in enumerate(occ_counts):
+        for w in range(_lib.OCCW):
+            k = min(64, max(0, c - 64 * w))
+            occ[i, w] = np.uint64((1 << k) - 1)
+    s.s_occ = occ
+    return s
+
+
+def test_quotas_are_multiples_of_the_recompute_interval():
+    s = _stub(100, 8, [0, 10, 50, 90, 100])
+    q = serving.Server.quotas(s, np.arange(5))
+    assert (q % 1024 == 0).all() and (q >= 1024).all() and (q <= 8 * 8192).all()
+    # a larger walked set costs more per hop, so it gets fewer hops
+    assert q[2] <= q[1] and q[2] <= q[3]
+
+
+def test_quotas_scale_with_site_chunks():
+    small = serving.Server.quotas(_stub(40, 8, [36]), np.arange(1))
+    large = serving.Server.quotas(_stub(100, 8, [90]), np.arange(1))
+    assert large[0] < small[0]
+
+
+def test_autotune_candidates_respect_lds():
+    c64 = set(autotune.candidates(64, 56, True))
+    assert {(1, 4), (2, 4), (3, 4)} <= c64
+    c128 = set(autotune.candidates(128, 120, True))
+    assert (1, 1) not in c128 and (2, 1) in c128          # {lt,b}+c table exceeds 160 KiB
+    c192 = set(autotune.candidates(192, 180, False))
+    assert c192 == set()                                   # no table fits, no positions
+    assert autotune.bucket_of(64) == 0 and autotune.bucket_of(65) == 1
+    with pytest.raises(ValueError):
+        autotune.bucket_of(193)
+
+
+def test_autotune_cache_roundtrip(tmp_path):
+    path = str(tmp_path / "c.json")
+    autotune._save_cache(path, {"k": {"form": 3, "waves": 4}})
+    assert autotune._load_cache(path) == {"k": {"form": 3, "waves": 4}}
+    assert autotune._load_cache(str(tmp_path / "missing.json")) == {}
+    assert autotune.cache_key("gfx950", 64, 1.0) != autotune.cache_key("gfx950", 64, 64.0)
