@@ -170,6 +170,7 @@ def serving_leg(args) -> dict:
             d.update(offered_req_per_s=rate, requests=len(sreq), rounds=srv.rounds - r0,
                      offered_hops_per_s=hops / sreq[-1].t_arrive,
                      kernel_ms_total=float(np.sum(srv.round_ms[r0:])),
+                     round_ms_mean=float(np.mean(srv.round_ms[r0:])),
                      host_s={k: round(v, 3) for k, v in srv.host_s.items()})
             for k in srv.host_s:
                 srv.host_s[k] = 0.0

@@ -190,6 +190,9 @@ int kmc_batch_sync(int32_t batch, double* elapsed_ms);
  * hops_done[ntraj], occ_time[ntraj*KMC_NMAX] (only after a record run). */
 int kmc_batch_results(int32_t batch, double* time, int64_t* eo, uint64_t* occ, int32_t* status,
                       int64_t* hops_done, double* occ_time);
+/* Device time of each trajectory in the last FP32 run (caller order, seconds,
+ * from the GPU's constant-rate counter): what a scheduler sizes hop ranges by. */
+int kmc_batch_timing(int32_t batch, double* seconds);
 /* Net traffic of the last FP32 run with record bit 1: traffic[ntraj*stride],
  * trajectory j's S*S matrix at j*stride (stride returned in *stride_out). */
 int kmc_batch_traffic(int32_t batch, double* traffic, int64_t* stride_out);

@@ -87,6 +87,7 @@ SIGNATURES = {
     "kmc_batch_sync": (C.c_int, [_i32, _dp]),
     "kmc_batch_results": (C.c_int, [_i32, _dp, _i64p, _u64p, _i32p, _i64p, _dp]),
     "kmc_batch_traffic": (C.c_int, [_i32, _dp, _i64p]),
+    "kmc_batch_timing": (C.c_int, [_i32, _dp]),
     "kmc_batch_destroy": (C.c_int, [_i32]),
     "kmc_probe_copy": (C.c_int, [_i64, C.c_int, _dp]),
     "kmc_probe_lds": (C.c_int, [C.c_int, C.c_int, C.c_int, _dp]),

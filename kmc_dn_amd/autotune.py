@@ -32,6 +32,9 @@ BUCKET_EDGES = (64, 128, 192)
 FORMS = (1, 2, 3)
 WAVES = (1, 2, 4, 8)
 LDS_BYTES = 160 * 1024
+# concurrent bucket launches: HIP gives a process 4 hardware queues by default and
+# the default stream (host copies) holds one of them
+MAX_LAUNCHES = 3
 CACHE_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "autotune_cache.json")
 
 
@@ -186,7 +189,7 @@ def plan_for(trajs: np.ndarray, sizes: dict, has_pos: bool = True, edges=BUCKET_
         rate = cache.get(cache_key(device, edge, tpl), {}).get("hops_per_s")
         if rate:
             t_thr = max(t_thr, float(budget[sel].sum()) / rate)
-    plan = []
+    teams, solos = [], []
     for edge, form, w, _ in solo:
         b = edges.index(edge)
         lt = lat.get(edge)
@@ -201,9 +204,60 @@ def plan_for(trajs: np.ndarray, sizes: dict, has_pos: bool = True, edges=BUCKET_
                 if thr is not None and not (budget[sel] >= thr).any():
                     thr = None
             if thr is not None:
-                plan.append((edge, int(lt["form"]), 1, int(lt["team"]), int(thr)))
-        plan.append((edge, form, w, 1, 0))
-    return plan
+                teams.append((edge, int(lt["form"]), 1, int(lt["team"]), int(thr)))
+        solos.append((edge, form, w, 1, 0))
+    N_of = np.array([sizes[int(x)][1] for x in lay])
+    max_sn = {e: (int(S_of[bucket == b].max()), int(N_of[bucket == b].max()))
+              for b, e in enumerate(edges) if (bucket == b).any()}
+    return merge_plan(teams, solos, max_sn, has_pos)
+
+
+def merge_plan(teams: list, solos: list, max_sn: dict, has_pos: bool = True,
+               max_launches: int = MAX_LAUNCHES) -> list:
+    """Team entries (by size) then solo entries (by size); while there are more
+    than `max_launches`, the two largest-size solo entries merge, else the two
+    largest-size team entries (while the merged table still fits LDS).  A merged
+    entry keeps the larger size's kernel (its site-chunk count covers both
+    sizes) and the smaller team threshold; a merged solo entry uses the
+    positions form (per-wave coordinates, no per-layout table), which serves
+    every size.  Small-size team entries stay apart longest: they hold the long
+    trajectories whose hop latency the larger kernels would raise.  Each entry
+    is one concurrent launch: HIP runs launches whose streams share a hardware
+    queue one after the other."""
+    teams, solos = sorted(teams), sorted(solos)
+    sn = dict(max_sn)
+    red = 2 * 4 * 3 * 64 * 4
+
+    def combined(a, b):
+        S = max(sn.get(a[0], (0, 0))[0], sn.get(b[0], (0, 0))[0])
+        N = max(sn.get(a[0], (0, 0))[1], sn.get(b[0], (0, 0))[1])
+        return S, N
+
+    while len(teams) + len(solos) > max_launches:
+        merged = None
+        if len(solos) >= 2:
+            a, b = solos[-2], solos[-1]
+            S, N = combined(a, b)
+            if has_pos:
+                merged = (solos, (b[0], 3, 4, 1, 0), S, N)
+            elif table_lds_bytes(b[1], S, N) <= LDS_BYTES:
+                merged = (solos, (b[0], b[1], b[2], 1, 0), S, N)
+        if merged is None and len(teams) >= 2:
+            a, b = teams[-2], teams[-1]
+            S, N = combined(a, b)
+            form = max(a[1], b[1]) if a[1] != 3 and b[1] != 3 else 3
+            if form == 1 and table_lds_bytes(1, S, N) + red > LDS_BYTES:
+                form = 2
+            if form != 3 and table_lds_bytes(form, S, N) + red > LDS_BYTES:
+                form = 3 if has_pos else None
+            if form is not None:
+                merged = (teams, (b[0], form, 1, max(a[3], b[3]), min(a[4], b[4])), S, N)
+        if merged is None:
+            break
+        kind, entry, S, N = merged
+        kind[-2:] = [entry]
+        sn[entry[0]] = (S, N)
+    return teams + solos
 
 
 def tune(trajs: np.ndarray, sizes: dict, has_pos: bool = True, sample: int = 1024,

@@ -893,9 +893,10 @@ int launch_one(const Fp32Launch& L, hipStream_t s) {
   return KMC_OK;
 }
 
+// Side streams 1..n-1 (launch 0 runs on the engine stream).
 int ensure_side_streams(Batch* B, int n) {
   if (!B->fork) HIPOK(hipEventCreateWithFlags(&B->fork, hipEventDisableTiming));
-  for (int b = 0; b < n; ++b) {
+  for (int b = 1; b < n; ++b) {
     if (!B->side[b]) HIPOK(hipStreamCreateWithFlags(&B->side[b], hipStreamNonBlocking));
     if (!B->join[b]) HIPOK(hipEventCreateWithFlags(&B->join[b], hipEventDisableTiming));
   }
@@ -908,14 +909,19 @@ int launch_fp32(Batch* B, const kmc_run_opts& o, const std::vector<Fp32Launch>& 
   if ((o.record & 2) && !o.resume)
     HIPOK(hipMemsetAsync(B->traffic.p, 0, sizeof(double) * (size_t)B->smax * B->smax * B->ntraj, s));
   if (Ls.size() == 1) return launch_one(Ls[0], s);
+  // launch 0 stays on s, the others fork onto side streams: n launches use n
+  // streams (HIP maps streams onto a few hardware queues; launches whose
+  // streams share a queue run one after the other)
   HIPOK(hipEventRecord(B->fork, s));
-  for (size_t b = 0; b < Ls.size(); ++b) {
+  for (size_t b = 1; b < Ls.size(); ++b) {
     HIPOK(hipStreamWaitEvent(B->side[b], B->fork, 0));
     int st = launch_one(Ls[b], B->side[b]);
     if (st) return st;
     HIPOK(hipEventRecord(B->join[b], B->side[b]));
   }
-  for (size_t b = 0; b < Ls.size(); ++b) HIPOK(hipStreamWaitEvent(s, B->join[b], 0));
+  int st = launch_one(Ls[0], s);
+  if (st) return st;
+  for (size_t b = 1; b < Ls.size(); ++b) HIPOK(hipStreamWaitEvent(s, B->join[b], 0));
   return KMC_OK;
 }
 
@@ -1226,6 +1232,22 @@ int kmc_batch_results(int32_t batch, double* time, int64_t* eo, uint64_t* occ, i
       for (int e = 0; e < PMAX; ++e) eo[j * PMAX + e] = e < P ? r.eo[e] : 0;
     if (occ_time) memcpy(occ_time + (size_t)j * NMAX, ot.data() + (size_t)i * NMAX, sizeof(double) * NMAX);
   }
+  return KMC_OK;
+}
+
+int kmc_batch_timing(int32_t batch, double* seconds) {
+  Engine& E = eng();
+  std::lock_guard<std::mutex> lk(E.mu);
+  Batch* B = get_batch(batch);
+  if (!B) return fail(KMC_ERR_ARGS, "bad batch id %d", batch);
+  if (!seconds) return fail(KMC_ERR_ARGS, "null output");
+  int khz = 0;
+  HIPOK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, E.device));
+  if (khz <= 0) return fail(KMC_ERR_DEVICE, "no wall-clock rate");
+  HIPOK(hipStreamSynchronize(E.stream));
+  std::vector<TrajOut> out(B->ntraj);
+  HIPOK(hipMemcpy(out.data(), B->out.p, sizeof(TrajOut) * B->ntraj, hipMemcpyDeviceToHost));
+  for (int64_t i = 0; i < B->ntraj; ++i) seconds[B->perm[i]] = out[i].ticks / (khz * 1e3);
   return KMC_OK;
 }
 

@@ -239,6 +239,7 @@ __device__ __forceinline__ float pick_lane_value(const float (&x)[C], int site) 
 template <int C, bool REC, class Pol, int T = 1>
 __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, const Pol& pol,
                                int lane, const Team<T, C>& team = Team<T, C>{}) {
+  const int64_t t_start = wall_clock64();
   const TrajDev td = p.trajs[t];
   const EsetDev es = p.esets[td.eset];
   const int N = L.N, S = L.S;
@@ -436,6 +437,8 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
 
   if (!team.writer()) return;
   if (lane == 0) {
+    const int64_t dt_ticks = wall_clock64() - t_start;
+    o->ticks = (int32_t)(dt_ticks < INT32_MAX ? dt_ticks : INT32_MAX);
     o->time = time;
     o->hops_done = h > td.prehops ? h - td.prehops : 0;
     o->status = status;

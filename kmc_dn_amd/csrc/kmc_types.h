@@ -68,7 +68,7 @@ struct TrajOut {
   double time;
   int64_t hops_done;
   int32_t status;
-  int32_t pad;
+  int32_t ticks;     // steady-counter ticks of the trajectory's last run (saturated)
   uint64_t occ[OCCW];
   int64_t eo[PMAX];
 };

@@ -220,6 +220,12 @@ class Batch:
                                            L.ptr(hd, L._i64p), L.ptr(ot, L._dp)))
         return BatchResult(t, eo, occ, st, hd, ot)
 
+    def timing(self) -> np.ndarray:
+        """Device seconds each trajectory took in the last FP32 run."""
+        sec = np.zeros(self.n)
+        L.check(L.load().kmc_batch_timing(self.id, L.ptr(sec, L._dp)))
+        return sec
+
     def traffic(self, sizes) -> list:
         """Net traffic of the last FP32 run with traffic=True: one S_j x S_j matrix per
         trajectory j, ``sizes[j]`` = S_j of its layout."""

@@ -99,20 +99,29 @@ class Server:
     """Continuous-batching evaluator over resident layouts.
 
     nets:  workloads.Network per layout; bases: laplace.LaplaceBasis per layout.
-    chunk_hops: hop range per round for a reference-cost trajectory (multiple of
-    the recompute interval); cheaper trajectories get proportionally more.
+    chunk_hops: sets the round's target device time, chunk_hops x 1.5 us; each
+      trajectory's hop range per round is that time over its measured seconds per
+      hop (a multiple of the recompute interval).
+    max_active / mem_budget: trajectory slots and device bytes of admitted state.
+    team_min_hops: budgets from which a request runs in a team bucket (None: solo
+      buckets only; every admitted trajectory must be resident in each round, and
+      table-form teams hold a CU's LDS each).
+    plan: fixed launch plan (else autotuned once at start-up).
     """
 
-    REF_COST = 32.0
+    REF_COST = 32.0           # cost-model units of a small network (one chunk, 16 walked)
+    REF_HOP_S = 1.5e-6        # first guess of the seconds per hop at REF_COST
 
     def __init__(self, nets, bases, chunk_hops: int = 8192, max_active: int = 8192,
                  mem_budget: int = 64 << 20, plan=None, recompute_interval: int = 1024,
-                 eset_block: int = 64, team_min_hops: int = 100_000, tune_log=None):
+                 eset_block: int = 64, team_min_hops: int | None = None, tune_log=None):
         if chunk_hops <= 0 or chunk_hops % recompute_interval:
             raise ValueError("chunk_hops must be a positive multiple of recompute_interval")
         self.nets, self.bases = list(nets), list(bases)
         self.K = recompute_interval
         self.chunk = chunk_hops
+        self.round_s = chunk_hops * self.REF_HOP_S      # target device time of a round
+        self.cal = self.REF_HOP_S / self.REF_COST       # seconds per hop per cost unit
         self.max_active = max_active
         self.mem_budget = mem_budget
         self.eset_block = eset_block
@@ -144,6 +153,7 @@ class Server:
         self.s_eo = np.zeros((n, L.PMAX), np.int64)
         self.s_occ = np.zeros((n, L.OCCW), np.uint64)
         self.s_status = np.zeros(n, np.int32)
+        self.s_lat = np.zeros(n)          # seconds per hop: measured, else modelled
         self.free_slots = list(range(n - 1, -1, -1))
         self.team_min_hops = team_min_hops
         self.latency = {}
@@ -157,6 +167,7 @@ class Server:
         self.completed = []
         self.rounds = 0
         self.round_ms = []
+        self.round_stats = []     # (trajectories, kernel ms, slowest and median trajectory ms)
         self.hops_done = 0
         self._next = 0
 
@@ -176,9 +187,13 @@ class Server:
             t["seed"][sl] = np.arange(reps)
             t["occ"][sl] = self.occ0[i]
         edges = autotune.BUCKET_EDGES
-        plan = autotune.plan_for(t, self.sizes, log=log,
-                                 team_min_hops={e: self.team_min_hops for e in edges})
-        self.latency = autotune.tune_latency(t, self.sizes, log=log)
+        if self.team_min_hops is None:   # throughput-tuned solo buckets only
+            plan = [tuple(e) for e in autotune.tune(t, self.sizes, log=log)]
+        else:
+            plan = autotune.plan_for(t, self.sizes, log=log,
+                                     team_min_hops={e: self.team_min_hops for e in edges})
+        if self.team_min_hops is not None:
+            self.latency = autotune.tune_latency(t, self.sizes, log=log)
         for i, es in taken:
             self._give_eset(i, es)
         return plan
@@ -284,6 +299,7 @@ class Server:
         self.s_eo[slots] = 0
         self.s_occ[slots] = self.occ0_table[lay]
         self.s_status[slots] = 0
+        self.s_lat[slots] = self.cost(slots) * self.cal
         off = 0
         for (req, _, ntask, need), sets in zip(cand, per_req_sets):
             self.active[self._next] = _Active(req, now, slots[off:off + ntask], sets, need)
@@ -310,18 +326,24 @@ class Server:
         self.batch = engine.Batch(t)
         self.batch.set_buckets(self.plan)
 
-    def quotas(self, idx: np.ndarray) -> np.ndarray:
-        """Hop range per trajectory this round: chunk_hops scaled by the inverse of a
-        per-hop cost model C * (16 + walked-set size), in multiples of the
-        recompute interval."""
+    def cost(self, idx: np.ndarray) -> np.ndarray:
+        """Static per-hop cost model, relative units: site chunks x (walked-set size
+        + fixed overhead), divided by the hop-latency gain of a team bucket."""
         li = self.s_lay[idx]
         N, P, C = self.lN[li], self.lP[li], self.lC[li]
         nocc = popcount_rows(self.s_occ[idx])
         walk = np.minimum(nocc + P, N - nocc + P)
         gain = self.team_gain(idx) if getattr(self, "latency", None) else 1.0
-        q = (self.chunk * self.REF_COST * gain / (C * (16 + walk))).astype(np.int64)
-        q = q // self.K * self.K
-        return np.clip(q, self.K, 8 * self.chunk)
+        return C * (16 + walk) / gain
+
+    def quotas(self, idx: np.ndarray) -> np.ndarray:
+        """Hop range per trajectory this round: the round's target time over the
+        trajectory's seconds per hop (measured in its previous round, else the cost
+        model times the current calibration), in multiples of the recompute
+        interval, so that every trajectory of a round ends at about the same time."""
+        lat = self.s_lat[idx]
+        q = (self.round_s / lat).astype(np.int64) // self.K * self.K
+        return np.clip(q, self.K, 16 * self.chunk)
 
     def plan_entry(self, S: np.ndarray, budget: np.ndarray) -> np.ndarray:
         """Index of the plan entry each trajectory runs in: the first entry with
@@ -370,6 +392,15 @@ class Server:
         self.round_ms.append(b.sync())
         t2 = time.perf_counter()
         r = b.results()
+        ran = he - hb
+        sec = b.timing()
+        m = (ran >= 256) & (sec > 0)
+        if m.any():
+            lat = sec[m] / ran[m]
+            self.s_lat[idx[m]] = lat
+            self.cal = float(np.median(lat / self.cost(idx[m])))
+        self.round_stats.append((n, self.round_ms[-1], float(sec.max()) * 1e3,
+                                 float(np.median(sec)) * 1e3))
         self.s_time[idx] = r.time
         self.s_eo[idx] = r.eo
         self.s_occ[idx] = r.occ_bits

@@ -24,7 +24,8 @@ def test_trajectory_seed_is_unique_per_request_and_slot():
 
 def _stub(N, P, occ_counts, chunk=8192, K=1024):
     n = len(occ_counts)
-    s = types.SimpleNamespace(chunk=chunk, K=K, REF_COST=serving.Server.REF_COST)
+    s = types.SimpleNamespace(chunk=chunk, K=K, REF_COST=serving.Server.REF_COST,
+                              round_s=chunk * serving.Server.REF_HOP_S)
     s.lN, s.lP = np.array([N]), np.array([P])
     s.lC = (s.lN + s.lP + 63) // 64
     s.s_lay = np.zeros(n, np.int64)
@@ -34,21 +35,28 @@ def _stub(N, P, occ_counts, chunk=8192, K=1024):
             k = min(64, max(0, c - 64 * w))
             occ[i, w] = np.uint64((1 << k) - 1)
     s.s_occ = occ
+    s.s_lat = serving.Server.cost(s, np.arange(n)) * serving.Server.REF_HOP_S / s.REF_COST
     return s
+
+
+def test_cost_model_grows_with_walked_set_and_chunks():
+    s = _stub(100, 8, [0, 10, 50, 90, 100])
+    c = serving.Server.cost(s, np.arange(5))
+    # the walked set is the smaller of sources and targets: largest at half filling
+    assert c[2] >= c[1] and c[2] >= c[3]
+    small = serving.Server.cost(_stub(40, 8, [36]), np.arange(1))
+    large = serving.Server.cost(_stub(100, 8, [90]), np.arange(1))
+    assert large[0] > small[0]
 
 
 def test_quotas_are_multiples_of_the_recompute_interval():
     s = _stub(100, 8, [0, 10, 50, 90, 100])
     q = serving.Server.quotas(s, np.arange(5))
-    assert (q % 1024 == 0).all() and (q >= 1024).all() and (q <= 8 * 8192).all()
-    # a larger walked set costs more per hop, so it gets fewer hops
-    assert q[2] <= q[1] and q[2] <= q[3]
-
-
-def test_quotas_scale_with_site_chunks():
-    small = serving.Server.quotas(_stub(40, 8, [36]), np.arange(1))
-    large = serving.Server.quotas(_stub(100, 8, [90]), np.arange(1))
-    assert large[0] < small[0]
+    assert (q % 1024 == 0).all() and (q >= 1024).all() and (q <= 16 * 8192).all()
+    # slower trajectories (measured) get fewer hops for the same round time
+    s.s_lat = np.array([1e-6, 2e-6, 4e-6, 1e-5, 1e-3])
+    q = serving.Server.quotas(s, np.arange(5))
+    assert list(q) == sorted(q, reverse=True) and q[-1] == 1024
 
 
 def test_autotune_candidates_respect_lds():
@@ -69,3 +77,25 @@ def test_autotune_cache_roundtrip(tmp_path):
     assert autotune._load_cache(path) == {"k": {"form": 3, "waves": 4}}
     assert autotune._load_cache(str(tmp_path / "missing.json")) == {}
     assert autotune.cache_key("gfx950", 64, 1.0) != autotune.cache_key("gfx950", 64, 64.0)
+
+
+def test_merge_plan_fits_the_launch_limit():
+    teams = [(64, 1, 1, 4, 900_000), (128, 2, 1, 4, 270_000), (192, 2, 1, 4, 340_000)]
+    solos = [(64, 2, 4, 1, 0), (128, 3, 1, 1, 0), (192, 2, 8, 1, 0)]
+    sizes = {64: (64, 56), 128: (128, 116), 192: (132, 120)}
+    plan = autotune.merge_plan(teams, solos, sizes)
+    assert len(plan) <= autotune.MAX_LAUNCHES
+    # solo entries merge first into one positions-form entry for every size, then
+    # the larger team entries into one table-form kernel whose table still fits
+    assert plan == [(64, 1, 1, 4, 900_000), (192, 2, 1, 4, 270_000), (192, 3, 4, 1, 0)]
+    assert autotune.table_lds_bytes(2, 132, 120) + 2 * 4 * 3 * 64 * 4 <= autotune.LDS_BYTES
+    # first match: every (S, budget) has an entry; long ones land in team entries
+    for S in (15, 64, 65, 128, 132):
+        for budget in (10_000, 1_000_000):
+            hit = [e for e in plan if S <= e[0] and budget >= e[4]][0]
+            assert (hit[3] > 1) == (budget >= (900_000 if S <= 64 else 270_000))
+    small = autotune.merge_plan(teams[:1], solos[:2], {64: (64, 56), 128: (100, 90)})
+    assert small == teams[:1] + solos[:2]
+    four = autotune.merge_plan(teams[:2], solos[:2], {64: (64, 56), 128: (108, 100)},
+                               max_launches=4)
+    assert four == teams[:2] + solos[:2]
