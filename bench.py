@@ -33,12 +33,14 @@ if ROOT not in sys.path:
 
 METRIC = "hops/sec, 4096-traj N=30 ensemble vs plain; p50/p99 latency of served IV jobs"
 # MI355X: 256 CUs at up to 2.4 GHz; CDNA4's LDS (160 KiB per CU) delivers 256 bytes per
-# clock per CU (twice CDNA3's 128) -> 256 * 256 * 2.4e9 B/s = 157.3 TB/s.
-LDS_PEAK_GBS = 256 * 256 * 2.4
-# HBM bytes per launch of the batched kernel measured with rocprofv3 --pmc FETCH_SIZE
-# and WRITE_SIZE on this workload, scaled by the copy-kernel calibration factor
-# (profiles/kmc_batch_fp32_c1_hbm_traffic.md); None until measured.
-HBM_TRAFFIC_BYTES = None
+# clock per CU (twice CDNA3's 128) -> 256 * 256 * 2.4e9 B/s = 157.3 TB/s.  The roofline
+# divides by the LDS read peak measured in the same run (ds_read_b128 stream probe).
+LDS_PEAK_SPEC_GBS = 256 * 256 * 2.4
+# HBM bytes per launch of the batched kernel on the default workload (4096 x 1e5 hops,
+# N=30/P=8, auto table form and workgroup size): rocprofv3 --pmc FETCH_SIZE 379.125 KiB
+# and WRITE_SIZE 640 KiB per dispatch, scaled by the copy-kernel calibration factors
+# 2.0 (FETCH) and 1.0 (WRITE): (2 * 379.125 + 640) * 1024 bytes.
+HBM_TRAFFIC_BYTES = 1_431_808
 
 
 def lds_bytes_per_hop(N: int, P: int) -> int:
@@ -142,7 +144,7 @@ def uneven_leg(args) -> dict:
             "hops": run.hops, "networks": args.uneven_n, "plan": [list(p) for p in run.plan],
             "S_range": [int(S.min()), int(S.max())],
             "longest_budget": int(ub.hops[longest]), "longest_S": int(S[longest]),
-            "lds_roofline_frac": achieved / LDS_PEAK_GBS,
+            "lds_roofline_frac": achieved / LDS_PEAK_SPEC_GBS,
             "setup_s": setup, "tune_s": tune_s,
             "workload": "2048 seeded 3D networks in 1x1x0.2, N~U{10..120}, M=round(N/10), "
                         "P~U{4..12} side-face electrodes, V~U(-1,1), hops log-uniform [1e4,1e6]; "
@@ -250,6 +252,9 @@ def main() -> int:
     value = hops_per_step * args.steps / wall
     avg_kernel_s = float(np.mean(kernel_ms)) * 1e-3
     achieved = lds_bytes_per_hop(net.N, net.P) * hops_per_step / avg_kernel_s / 1e9
+    from kmc_dn_amd import probe
+    lds_peak = probe.lds(0)["GBps"]
+    default_workload = (args.ntraj, args.hops, args.waves_per_group) == (4096, 100_000, 0)
     cur = res.currents(net.P)
 
     # plain FP64 path: one trajectory per launch
@@ -289,12 +294,18 @@ def main() -> int:
         "roofline": {
             "bound": "lds",
             "achieved": achieved,
-            "peak": LDS_PEAK_GBS,
+            "peak": lds_peak,
             "unit": "GB/s",
-            "frac": achieved / LDS_PEAK_GBS,
-            "traffic": HBM_TRAFFIC_BYTES,
+            "frac": achieved / lds_peak,
+            "traffic": HBM_TRAFFIC_BYTES if default_workload else None,
+            "peak_spec": LDS_PEAK_SPEC_GBS,
+            "frac_spec": achieved / LDS_PEAK_SPEC_GBS,
             "bytes_per_hop": lds_bytes_per_hop(net.N, net.P),
+            "hops_per_launch": hops_per_step,
             "kernel_ms": float(np.mean(kernel_ms)),
+            "note": "algorithmic LDS bytes of the dense formulation / kernel time; peak = "
+                    "ds_read_b128 stream measured in this run; the kernel is bound by VALU "
+                    "issue (VALUBusy 72%, LdsUtil 9%)",
         },
         "plain_fp64": {"value": plain_value, "unit": "hops/s",
                        "kernel_ms": float(np.mean(pms)),
@@ -309,6 +320,9 @@ def main() -> int:
         out["uneven_batch"] = uneven_leg(args)
     if not args.no_serve:
         out["serving"] = serving_leg(args)
+        out["latency_ms_p50_p99"] = {k: [v.get("p50_ms"), v.get("p99_ms")]
+                                     for k, v in out["serving"].items()
+                                     if isinstance(v, dict) and "p50_ms" in v}
     if not args.no_cpu_baseline:
         numba, go = cpu_baselines(ens, args.cpu_seconds)
         out["cpu_baseline"] = numba

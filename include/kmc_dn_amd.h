@@ -262,7 +262,9 @@ int64_t parallelSimulations(kmc_GoSlice NSites, kmc_GoSlice NElectrodes, kmc_GoS
  * kernel whose FETCH_SIZE/WRITE_SIZE calibrates rocprof's HBM byte counters. */
 int kmc_probe_copy(int64_t bytes, int repeats, double* ms_per_copy);
 /* LDS read bandwidth (bytes/s) of `blocks` x 256 threads streaming
- * width_bytes (4, 8 or 16) reads for `iters` x 8 rounds. */
+ * width_bytes (4, 8 or 16) strided reads for `iters` x 8 rounds, or with
+ * width_bytes 0 contiguous ds_read_b128 with immediate offsets, 16 per lane and
+ * iteration (the LDS peak figure). */
 int kmc_probe_lds(int width_bytes, int blocks, int iters, double* bytes_per_s);
 
 #ifdef __cplusplus

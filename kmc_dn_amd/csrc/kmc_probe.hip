@@ -17,6 +17,30 @@ __global__ void __launch_bounds__(256) kmc_probe_copy_kernel(const float4* __res
     dst[i] = src[i];
 }
 
+// ds_read_b128 stream: each lane reads 16 B at a fixed base plus 16 immediate
+// offsets 4 KiB apart (a wave reads 1 KiB contiguous per load, no bank
+// conflicts, no address arithmetic); the empty asm makes the base opaque so the
+// loads are issued in every iteration.
+__global__ void __launch_bounds__(256) kmc_probe_lds_b128_kernel(int iters, uint32_t* out) {
+  __shared__ __align__(16) uint4 buf[4096];   // 64 KiB
+  for (int i = threadIdx.x; i < 4096; i += 256) buf[i] = make_uint4(i, 3 * i, 5 * i, 7 * i);
+  __syncthreads();
+  uint4 acc = make_uint4(0, 0, 0, 0);
+  int base = threadIdx.x;
+  for (int it = 0; it < iters; ++it) {
+    asm volatile("" : "+v"(base));   // an index, so the loads stay LDS loads
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      const uint4 v = buf[base + k * 256];
+      acc.x += v.x;
+      acc.y += v.y;
+      acc.z += v.z;
+      acc.w += v.w;
+    }
+  }
+  if ((acc.x ^ acc.y ^ acc.z ^ acc.w) == 0x12345678u) out[blockIdx.x] = acc.x;   // keeps the loads live
+}
+
 // Each thread streams WIDTH-byte LDS reads over a 64 KiB buffer `iters` times.
 template <int WIDTH>
 __global__ void __launch_bounds__(256) kmc_probe_lds_kernel(int iters, uint32_t* out) {
@@ -84,10 +108,12 @@ int kmc_probe_lds(int width_bytes, int blocks, int iters, double* bytes_per_s) {
   if (blocks <= 0 || iters <= 0) return KMC_ERR_ARGS;
   uint32_t* out = nullptr;
   if (hipMalloc(&out, sizeof(uint32_t) * blocks) != hipSuccess) return KMC_ERR_NOMEM;
-  void (*k)(int, uint32_t*) = width_bytes == 16 ? kmc_probe_lds_kernel<16>
-                              : width_bytes == 8 ? kmc_probe_lds_kernel<8>
-                                                 : kmc_probe_lds_kernel<4>;
-  const int w = width_bytes == 16 ? 16 : (width_bytes == 8 ? 8 : 4);
+  // width 0: the b128 stream (16 loads of 16 B per lane and iteration)
+  void (*k)(int, uint32_t*) = width_bytes == 0    ? kmc_probe_lds_b128_kernel
+                              : width_bytes == 16 ? kmc_probe_lds_kernel<16>
+                              : width_bytes == 8  ? kmc_probe_lds_kernel<8>
+                                                  : kmc_probe_lds_kernel<4>;
+  const double per_iter = width_bytes == 0 ? 16.0 * 16 : 8.0 * (width_bytes == 16 ? 16 : (width_bytes == 8 ? 8 : 4));
   hipEvent_t e0, e1;
   (void)hipEventCreate(&e0);
   (void)hipEventCreate(&e1);
@@ -98,7 +124,7 @@ int kmc_probe_lds(int width_bytes, int blocks, int iters, double* bytes_per_s) {
   hipError_t st = hipEventSynchronize(e1);
   float ms = 0.f;
   (void)hipEventElapsedTime(&ms, e0, e1);
-  if (bytes_per_s) *bytes_per_s = (double)blocks * 256 * 8 * (double)iters * w / (ms * 1e-3);
+  if (bytes_per_s) *bytes_per_s = (double)blocks * 256 * per_iter * (double)iters / (ms * 1e-3);
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   (void)hipFree(out);

@@ -2,7 +2,8 @@
 
 copy: a plain device copy of a known byte count -- the reference kernel for
       scaling rocprof's FETCH_SIZE / WRITE_SIZE to bytes;
-lds:  LDS read bandwidth of streaming ds_read_b32/b64/b128.
+lds:  LDS read bandwidth of strided ds_read_b32/b64/b128 (--width 4/8/16) and of
+      the ds_read_b128 stream with immediate offsets (--width 0, the peak figure).
 """
 from __future__ import annotations
 
@@ -32,13 +33,13 @@ def main() -> None:
     ap.add_argument("what", choices=["copy", "lds"])
     ap.add_argument("--bytes", type=int, default=1 << 30)
     ap.add_argument("--repeats", type=int, default=5)
-    ap.add_argument("--width", type=int, default=16)
+    ap.add_argument("--width", type=int, default=-1, help="4, 8, 16, 0 (stream); -1 all")
     args = ap.parse_args()
     engine.init(0)
     if args.what == "copy":
         print(json.dumps(copy(args.bytes, args.repeats)))
     else:
-        for w in (4, 8, 16) if args.width == 0 else (args.width,):
+        for w in (4, 8, 16, 0) if args.width < 0 else (args.width,):
             print(json.dumps(lds(w)))
 
 
