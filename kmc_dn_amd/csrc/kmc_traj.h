@@ -66,6 +66,25 @@ __device__ __forceinline__ void recompute_energies(float (&se)[C], const uint64_
   }
 }
 
+// Lanes l of site chunk c (site l + 64c) with lo <= site < hi, as a lane mask.
+__device__ __forceinline__ uint64_t chunk_mask(int lo, int hi, int c) {
+  const int a = lo - 64 * c < 0 ? 0 : lo - 64 * c;
+  const int b = hi - 64 * c > 64 ? 64 : hi - 64 * c;
+  if (b <= a) return 0ull;
+  const uint64_t below_b = b >= 64 ? ~0ull : ((1ull << b) - 1);
+  return below_b & ~((1ull << a) - 1);
+}
+
+// min(a, b) for non-NaN operands as one v_min_f32.  fminf (and fmed3 with an
+// infinite operand, which the compiler folds into it) first canonicalises any
+// operand loaded from LDS with an extra v_max_f32 under IEEE mode; the pair
+// terms are never NaN, so the plain instruction is exact here.
+__device__ __forceinline__ float min_f32(float a, float b) {
+  float r;
+  asm("v_min_f32 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+  return r;
+}
+
 // ---------------------------------------------------------------------------
 // Pair-term policies.  For a uniform member u of the walked set (u < 0: none)
 // and the lane sites s = lane + 64c, pair(pre(u), c, diff) is log2 of the rate
@@ -89,8 +108,8 @@ struct TabPolicy {
     return p;
   }
   __device__ __forceinline__ float pair(const Pre& p, int c, float diff) const {
-    if constexpr (BT) return fminf(p.e[c].x, p.e[c].y + diff);
-    else return p.e[c].x + fminf(0.f, diff + p.e[c].y);
+    if constexpr (BT) return min_f32(p.e[c].x, p.e[c].y + diff);
+    else return p.e[c].x + min_f32(0.f, diff + p.e[c].y);
   }
   __device__ __forceinline__ float coul(int u, int c) const {
     if constexpr (BT) return ctab[u * S + lane + 64 * c];
@@ -102,15 +121,17 @@ template <int C>
 struct PosPolicy {
   const float4* pos;   // [S] per wave: x, y, z, 1 for acceptors / 0 for electrodes
   float px[C], py[C], pz[C], kcl[C];   // the lane sites' coordinates and c scale
-  bool lane_el[C];
+  uint64_t elm[C];     // lanes of chunk c that hold electrodes
   float l0, mkl;       // lt = l0 + mkl * d
-  int lane;
+  int lane, N;
   struct Pre {
     float4 q;
+    bool pad, el;      // wave-uniform: no member / the member is an electrode
   };
   __device__ __forceinline__ void init(const float4* p, const LayoutDev& L, int ln) {
     pos = p;
     lane = ln;
+    N = L.N;
     l0 = L.pos_l0;
     mkl = L.pos_mkl;
 #pragma unroll
@@ -121,12 +142,14 @@ struct PosPolicy {
       py[c] = q.y;
       pz[c] = q.z;
       kcl[c] = q.w * L.pos_kc;
-      lane_el[c] = s >= L.N && s < L.S;
+      elm[c] = chunk_mask(L.N, L.S, c);
     }
   }
   __device__ __forceinline__ Pre prefetch(int u) const {
     Pre p;
-    p.q = u < 0 ? make_float4(0.f, 0.f, 0.f, -1.f) : pos[u];
+    p.q = u < 0 ? make_float4(0.f, 0.f, 0.f, 0.f) : pos[u];
+    p.pad = u < 0;
+    p.el = u >= N;
     return p;
   }
   __device__ __forceinline__ float dist_rinv(const float4& q, int c, float& rinv) const {
@@ -140,10 +163,11 @@ struct PosPolicy {
     const float d = dist_rinv(p.q, c, rinv);
     const float lt = fmaf(mkl, d, l0);
     const float cc = rinv * kcl[c] * p.q.w;
-    const float x = lt + fminf(0.f, diff + cc);
+    const float x = lt + min_f32(0.f, diff + cc);
     // padding members and electrode-electrode pairs (an electrode with itself
-    // included) have no rate
-    return (p.q.w < 0.f || (p.q.w == 0.f && lane_el[c])) ? -INFINITY : x;
+    // included) have no rate: a wave-uniform lane mask selects them
+    const uint64_t none = p.pad ? ~0ull : (p.el ? elm[c] : 0ull);
+    return __builtin_amdgcn_inverse_ballot_w64(none) ? -INFINITY : x;
   }
   __device__ __forceinline__ float coul(int u, int c) const {
     const float4 q = pos[u];
@@ -264,6 +288,12 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
   double time = cont ? o->time : 0.0;
   int32_t status = cont ? o->status : (int32_t)ST_OK;
   recompute_energies<C>(se, occ, L, es, lane);
+  uint64_t accm[C], elm[C];   // lanes of chunk c holding acceptors / electrodes
+#pragma unroll
+  for (int c = 0; c < C; ++c) {
+    accm[c] = chunk_mask(0, N, c);
+    elm[c] = chunk_mask(N, S, c);
+  }
 
   const int K = p.recompute_interval;
   float lg_lane = 0.f;    // log2(1 - u_clock) of hop (h & ~63) + lane
@@ -277,7 +307,7 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
       stream_uniforms(td.seed, td.hop0 + (uint64_t)((h & ~(int64_t)63) + lane), uc, ue_lane);
       lg_lane = fast_log2((float)(1.0 - uc));
     }
-    if (h == td.prehops) {
+    if (__builtin_expect(h == td.prehops, 0)) {
       time = 0.0;
 #pragma unroll
       for (int c = 0; c < C; ++c) {
@@ -290,28 +320,24 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
       next_rc += K;
     }
 
-    // roles of the lanes' sites
+    // roles of the lanes' sites, as wave-uniform lane masks (scalar ops only):
+    // sources = occupied acceptors + electrodes, targets = empty acceptors + electrodes
     uint64_t srcm[C], dstm[C];
-    bool occd[C];
     int nsrc = 0, ndst = 0;
 #pragma unroll
     for (int c = 0; c < C; ++c) {
-      const int s = lane + 64 * c;
-      const bool isacc = s < N;
-      const bool isel = s >= N && s < S;
-      occd[c] = isacc && ((occ[s >> 6] >> (s & 63)) & 1ull);
-      srcm[c] = ballot(isel || occd[c]);
-      dstm[c] = ballot(isel || (isacc && !occd[c]));
+      srcm[c] = (occ[c] & accm[c]) | elm[c];
+      dstm[c] = (~occ[c] & accm[c]) | elm[c];
       nsrc += __builtin_popcountll(srcm[c]);
       ndst += __builtin_popcountll(dstm[c]);
     }
     const bool udst = ndst <= nsrc;   // walk the smaller set
+    const float vfill = udst ? -INFINITY : INFINITY;
     float v[C], R[C];
 #pragma unroll
     for (int c = 0; c < C; ++c) {
       R[c] = 0.f;
-      const bool lane_ok = udst ? ((srcm[c] >> lane) & 1ull) : ((dstm[c] >> lane) & 1ull);
-      v[c] = lane_ok ? se[c] : (udst ? -INFINITY : INFINITY);
+      v[c] = __builtin_amdgcn_inverse_ballot_w64(udst ? srcm[c] : dstm[c]) ? se[c] : vfill;
     }
     if (udst) row_sums<C, true>(pol, S, dstm, team.own(), v, se, R);
     else row_sums<C, false>(pol, S, srcm, team.own(), v, se, R);
@@ -326,7 +352,7 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
         incl[c] = total;
         continue;
       }
-      incl[c] = wave_scan_f64((double)R[c]) + total;
+      incl[c] = c == 0 ? wave_scan_f64((double)R[c]) : wave_scan_f64((double)R[c]) + total;
       total = readlane_f64(incl[c], 63);
     }
     if (!(total > 0.0) || !(total < INFINITY)) {
@@ -365,14 +391,16 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
 
     // partner of `sel`: lane-parallel pass over its row, in site order
     const float s_sel = pick_lane_value<C>(se, sel);
+    const float sg = udst ? -1.f : 1.f;
     const typename Pol::Pre pre = pol.prefetch(sel);
     float run = 0.f;
     int pick = -1, last_nz = -1;
 #pragma unroll
     for (int c = 0; c < C; ++c) {
       if (c > 0 && 64 * c >= S) break;
-      const bool inU = udst ? ((dstm[c] >> lane) & 1ull) : ((srcm[c] >> lane) & 1ull);
-      const float diff = udst ? (s_sel - se[c]) : (se[c] - s_sel);
+      const bool inU = __builtin_amdgcn_inverse_ballot_w64(udst ? dstm[c] : srcm[c]);
+      // udst ? s_sel - se : se - s_sel, one FMA (the sign is exact)
+      const float diff = fmaf(sg, se[c], -sg * s_sel);
       float r = fast_exp2(pol.pair(pre, c, diff));
       r = inU ? r : 0.f;
       const float cum = wave_scan_f32(r) + run;
@@ -396,7 +424,7 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
     if (REC && tally && !p.post_hop_record) {
 #pragma unroll
       for (int c = 0; c < C; ++c)
-        if (occd[c]) ot[c] += dt;
+        if (__builtin_amdgcn_inverse_ballot_w64(occ[c] & accm[c])) ot[c] += dt;
     }
     // net traffic, Go convention (simulation.go:310-311)
     if (p.traffic && tally && lane == 0 && team.writer()) {
@@ -427,10 +455,8 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
     }
     if (REC && tally && p.post_hop_record) {
 #pragma unroll
-      for (int c = 0; c < C; ++c) {
-        const int s = lane + 64 * c;
-        if (s < N && ((occ[s >> 6] >> (s & 63)) & 1ull)) ot[c] += dt;
-      }
+      for (int c = 0; c < C; ++c)
+        if (__builtin_amdgcn_inverse_ballot_w64(occ[c] & accm[c])) ot[c] += dt;
     }
     time += dt;
   }
