@@ -93,7 +93,9 @@ int kmc_layout_create(int64_t N, int64_t P, double nu, double kT, double I_0, do
 /* Attaches site coordinates (S*3, acceptors then electrodes) and the localisation
  * length ab to a layout, enabling the positions form (table_form 3).  The layout's
  * distances and transitions_constant must equal the reference closed forms
- * (kmc_dopant_networks.py:657-695, :824-830) within 1e-9 relative. */
+ * (kmc_dopant_networks.py:657-695, :824-830) within 1e-9 relative.  Layouts
+ * created with prune_threshold > 0 are refused (KMC_ERR_UNSUPPORTED): the
+ * positions form evaluates every pair from the coordinates. */
 int kmc_layout_set_positions(int32_t layout, double ab, const double* positions);
 
 /* Frees a layout's tables and every energy set created on it (ids may be reused). */

@@ -359,6 +359,9 @@ int eset_create_locked(int32_t layout, int64_t count, const double* Ec, const do
 // these coordinates, then store float4 coordinates and the closed-form constants.
 int layout_set_positions_locked(Layout* L, double ab, const double* xyz) {
   const int S = L->S, n = L->N;
+  if (L->dev.prune)   // the positions form evaluates every pair from the coordinates
+    return fail(KMC_ERR_UNSUPPORTED, "the positions form does not apply a pruned pair set; "
+                                     "use the table forms for pruned layouts");
   std::vector<double> d((size_t)S * S), tc((size_t)S * S);
   HIPOK(hipMemcpy(tc.data(), L->tc64.p, sizeof(double) * S * S, hipMemcpyDeviceToHost));
   const double nu = L->dev.nu;

@@ -275,7 +275,6 @@ class kmc_dn:
         """kmc_dopant_networks.py:544-618: prehops (FP64, as the reference always uses numba),
         reset, hops, then current = electrode_occupation / time."""
         self.reset()
-        self.reset()
         if prehops != 0:
             self._fp64_hops(int(prehops), False)
             self.reset()

@@ -264,3 +264,34 @@ def test_budget_buckets_route_long_trajectories_to_teams(gpu_engine):
         assert np.array_equal(r.eo[sel], mixed.eo[sel]) and np.array_equal(r.time[sel], mixed.time[sel])
     b.close()
     lay.close()
+
+
+def test_pruned_pairs_never_hop(gpu_engine):
+    """wrapperSimulatePruned semantics (simulation.go:199-215): pairs with
+    tc <= prune * max(tc) are dropped from the FP32 pair set."""
+    net = random_network(21, 30, 8, I_0=50.0, a=0.25, vmax=1.0)
+    N, P, S = 30, 8, 38
+    tc = net["transitions_constant"]
+    prune = 1e-3
+    pruned = tc <= prune * tc.max()
+    np.fill_diagonal(pruned, False)
+    assert 0.05 < pruned.mean() < 0.95
+    lay = engine.Layout(N, P, net["nu"], net["kT"], net["I_0"], net["R"], net["distances"], tc,
+                        prune_threshold=prune)
+    with pytest.raises(_lib.KmcError) as e:
+        lay.set_positions(0.25 * net["R"], _positions(net))
+    assert e.value.code == _lib.ERR_UNSUPPORTED
+    es = lay.energy_sets(net["E_constant"], net["site_energies"][N:])
+    K = 64
+    b = engine.Batch(engine.make_trajs(K, lay.id, es[0], np.arange(K, dtype=np.uint64), 20_000,
+                                       net["occupation"]))
+    for form in (1, 2):
+        print(f"[gpu] pruned pair set, table form {form}", flush=True)
+        b.run(precision=_lib.FP32_BATCHED, table_form=form, traffic=True)
+        b.sync()
+        assert (b.results().status == 0).all()
+        tr = np.stack(b.traffic(S))          # net traffic, antisymmetric
+        moved = np.abs(tr).sum(0) > 0
+        assert moved.any() and not (moved & pruned).any()
+    b.close()
+    lay.close()

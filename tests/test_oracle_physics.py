@@ -89,3 +89,24 @@ def test_go_and_numba_restatements_agree():
     c1, c2 = eo1 / t1[:, None], eo2 / t2[:, None]
     se = np.sqrt(c1.var(0, ddof=1) / 16 + c2.var(0, ddof=1) / 16)
     assert np.all(np.abs(c1.mean(0) - c2.mean(0)) < 4.5 * se + 1e-9)
+
+
+def test_go_restatement_prunes_small_prefactors():
+    """simulation.go:199-215: pairs with tc <= prune * max(tc) leave the pair list,
+    so the logged (from, to) events never use one; with prune = 0 they do occur."""
+    from tests.netgen import random_network, oracle_args
+    net = random_network(21, 12, 4, I_0=50.0, a=0.4, vmax=1.0)
+    tc = net["transitions_constant"]
+    prune = 0.05
+    pruned = tc <= prune * tc.max()
+    np.fill_diagonal(pruned, False)
+    assert 0.05 < pruned.mean() < 0.95
+    args = oracle_args(net)
+    hits = {}
+    for p in (0.0, prune):
+        r = oracle.go_simulate(*args, occupation=net["occupation"], hops=20_000,
+                               prune_threshold=p, seed=3, log=True)
+        assert r.status == 0
+        ev = r.events
+        hits[p] = int(pruned[ev[:, 0], ev[:, 1]].sum())
+    assert hits[prune] == 0 and hits[0.0] > 0
