@@ -221,24 +221,20 @@ __device__ __forceinline__ void row_sums(const Pol& pol, int S, const uint64_t (
   for (int cu = 0; cu < C; ++cu) {
     uint64_t m = umask[cu] & own;
     while (m) {
+      // branch-free: an exhausted mask yields lane -1 (a padding member) and stays 0
       int b[U], uu[U];
 #pragma unroll
       for (int k = 0; k < U; ++k) {
-        if (m) {
-          b[k] = __builtin_ctzll(m);
-          m &= m - 1;
-          uu[k] = 64 * cu + b[k];
-        } else {
-          b[k] = 0;
-          uu[k] = -1;
-        }
+        b[k] = m ? __builtin_ctzll(m) : -1;
+        m &= m - 1;
+        uu[k] = b[k] < 0 ? -1 : 64 * cu + b[k];
       }
       typename Pol::Pre pre[U];
 #pragma unroll
       for (int k = 0; k < U; ++k) pre[k] = pol.prefetch(uu[k]);
 #pragma unroll
       for (int k = 0; k < U; ++k) {
-        const float su = readlane_f32(se[cu], b[k]);
+        const float su = readlane_f32(se[cu], b[k] & 63);
 #pragma unroll
         for (int c = 0; c < C; ++c) {
           if (c > 0 && 64 * c >= S) break;   // chunk past this layout's sites
