@@ -428,26 +428,23 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
       tr[src * S + dst] += 1.0;
       tr[dst * S + src] -= 1.0;
     }
-    // perform hop: occupation, rank-1 energy update, tallies
-    if (src < N) {
-      occ[src >> 6] &= ~(1ull << (src & 63));
+    // perform hop: occupation (acceptors live in the words of the C chunks),
+    // rank-1 energy update (both Coulomb columns loaded together; an electrode
+    // end contributes 0), tallies
+    const bool src_acc = src < N, dst_acc = dst < N;
+    float cs[C], cd[C];
 #pragma unroll
-      for (int c = 0; c < C; ++c)
-        if (c == 0 || 64 * c < S) se[c] -= pol.coul(src, c);
-    } else {
-#pragma unroll
-      for (int c = 0; c < C; ++c)
-        if (lane + 64 * c == src) eo[c] -= 1;
+    for (int c = 0; c < C; ++c) {
+      cs[c] = (c == 0 || 64 * c < S) ? pol.coul(src_acc ? src : 0, c) : 0.f;
+      cd[c] = (c == 0 || 64 * c < S) ? pol.coul(dst_acc ? dst : 0, c) : 0.f;
     }
-    if (dst < N) {
-      occ[dst >> 6] |= 1ull << (dst & 63);
 #pragma unroll
-      for (int c = 0; c < C; ++c)
-        if (c == 0 || 64 * c < S) se[c] += pol.coul(dst, c);
-    } else {
-#pragma unroll
-      for (int c = 0; c < C; ++c)
-        if (lane + 64 * c == dst) eo[c] += 1;
+    for (int c = 0; c < C; ++c) {
+      if (src_acc && (src >> 6) == c) occ[c] &= ~(1ull << (src & 63));
+      if (dst_acc && (dst >> 6) == c) occ[c] |= 1ull << (dst & 63);
+      se[c] = (se[c] - (src_acc ? cs[c] : 0.f)) + (dst_acc ? cd[c] : 0.f);
+      if (!src_acc && lane + 64 * c == src) eo[c] -= 1;
+      if (!dst_acc && lane + 64 * c == dst) eo[c] += 1;
     }
     if (REC && tally && p.post_hop_record) {
 #pragma unroll
