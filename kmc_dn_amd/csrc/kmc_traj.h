@@ -66,6 +66,20 @@ __device__ __forceinline__ void recompute_energies(float (&se)[C], const uint64_
   }
 }
 
+// Lowest set bit of a wave-uniform mask, -1 when it is empty (s_ff1 returns -1
+// for 0, which __builtin_ctzll leaves undefined).
+__device__ __forceinline__ int lowest_bit(uint64_t m) {
+  int r;
+  asm("s_ff1_i32_b64 %0, %1" : "=s"(r) : "s"(m));
+  return r;
+}
+
+// m with bit (b mod 64) cleared.
+__device__ __forceinline__ uint64_t clear_bit(uint64_t m, int b) {
+  asm("s_bitset0_b64 %0, %1" : "+s"(m) : "s"(b));
+  return m;
+}
+
 // Lanes l of site chunk c (site l + 64c) with lo <= site < hi, as a lane mask.
 __device__ __forceinline__ uint64_t chunk_mask(int lo, int hi, int c) {
   const int a = lo - 64 * c < 0 ? 0 : lo - 64 * c;
@@ -100,9 +114,10 @@ struct TabPolicy {
   struct Pre {
     float2 e[C];
   };
+  // u in [0, S): a member; u == S: padding (the -inf sentinel column at S*S)
   __device__ __forceinline__ Pre prefetch(int u) const {
     Pre p;
-    const float2* col = tab + (u < 0 ? S * S : u * S) + lane;
+    const float2* col = tab + u * S + lane;
 #pragma unroll
     for (int c = 0; c < C; ++c) p.e[c] = col[64 * c];
     return p;
@@ -123,7 +138,7 @@ struct PosPolicy {
   float px[C], py[C], pz[C], kcl[C];   // the lane sites' coordinates and c scale
   uint64_t elm[C];     // lanes of chunk c that hold electrodes
   float l0, mkl;       // lt = l0 + mkl * d
-  int lane, N;
+  int lane, N, S;
   struct Pre {
     float4 q;
     bool pad, el;      // wave-uniform: no member / the member is an electrode
@@ -132,6 +147,7 @@ struct PosPolicy {
     pos = p;
     lane = ln;
     N = L.N;
+    S = L.S;
     l0 = L.pos_l0;
     mkl = L.pos_mkl;
 #pragma unroll
@@ -145,10 +161,11 @@ struct PosPolicy {
       elm[c] = chunk_mask(L.N, L.S, c);
     }
   }
+  // u in [0, S): a member; u == S: padding
   __device__ __forceinline__ Pre prefetch(int u) const {
     Pre p;
-    p.q = u < 0 ? make_float4(0.f, 0.f, 0.f, 0.f) : pos[u];
-    p.pad = u < 0;
+    p.pad = u >= S;
+    p.q = pos[p.pad ? 0 : u];
     p.el = u >= N;
     return p;
   }
@@ -221,13 +238,15 @@ __device__ __forceinline__ void row_sums(const Pol& pol, int S, const uint64_t (
   for (int cu = 0; cu < C; ++cu) {
     uint64_t m = umask[cu] & own;
     while (m) {
-      // branch-free: an exhausted mask yields lane -1 (a padding member) and stays 0
+      // branch-free: an exhausted mask yields lane -1, i.e. member S (padding),
+      // and stays empty
       int b[U], uu[U];
 #pragma unroll
       for (int k = 0; k < U; ++k) {
-        b[k] = m ? __builtin_ctzll(m) : -1;
-        m &= m - 1;
-        uu[k] = b[k] < 0 ? -1 : 64 * cu + b[k];
+        b[k] = lowest_bit(m);
+        m = clear_bit(m, b[k]);
+        uu[k] = cu == 0 ? (int)min((unsigned)b[k], (unsigned)S)
+                        : (b[k] < 0 ? S : 64 * cu + b[k]);
       }
       typename Pol::Pre pre[U];
 #pragma unroll

@@ -54,7 +54,7 @@ def ensemble_check(K=4096, hops=100_000):
             "electrode_events_per_traj": float(np.abs(r64.eo[:, :n.P]).sum(1).mean())}
 
 
-def fixtures_check(n_layouts=10, replicas=32, hops=1_000_000, path=None):
+def fixtures_check(n_layouts=10, replicas=128, hops=1_000_000, path=None):
     if path is None:
         path = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
                             "tests", "golden", "kmc_fixtures.npz")
@@ -101,7 +101,7 @@ def main() -> None:
     a = ap.parse_args()
     engine.init(0)
     out = {"ensemble": ensemble_check(1024 if a.quick else 4096, 20000 if a.quick else 100_000),
-           "fixtures": fixtures_check(4 if a.quick else 10, 8 if a.quick else 32,
+           "fixtures": fixtures_check(4 if a.quick else 10, 8 if a.quick else 128,
                                       100_000 if a.quick else 1_000_000)}
     print(json.dumps(out))
 
