@@ -38,11 +38,15 @@ MAX_LAUNCHES = 3
 CACHE_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "autotune_cache.json")
 
 
-def table_lds_bytes(form: int, S: int, N: int) -> int:
-    """LDS bytes of a table-form workgroup for S sites, N acceptors."""
-    b = (S * S + 192) * 8
+def table_lds_bytes(form: int, S: int, N: int, waves: int = 8) -> int:
+    """Upper bound of the LDS bytes of a table-form workgroup of `waves` waves
+    (or a team of as many) for layouts of at most S sites and N acceptors: the
+    waves' electrode factors, the electrode weights of up to 16 electrodes, the
+    pair table and, for form 1, the c table (kmc_batch.hip batch_lds_bytes)."""
+    C = (S + 63) // 64
+    b = waves * 2 * 16 * 4 + ((16 * S + 64 * C + 3) & ~3) * 4 + (S * S + 64 * C) * 8
     if form == 1:
-        b += (N * S + 192) * 4
+        b += (N * S + 64 * C) * 4
     return b
 
 
@@ -57,9 +61,9 @@ def candidates(smax: int, nmax: int, has_pos: bool):
     for form in FORMS:
         if form == 3 and not has_pos:
             continue
-        if form != 3 and table_lds_bytes(form, smax, nmax) > LDS_BYTES:
-            continue
         for w in WAVES:
+            if form != 3 and table_lds_bytes(form, smax, nmax, w) > LDS_BYTES:
+                continue
             yield form, w
 
 

@@ -8,8 +8,9 @@
 namespace kmc {
 
 // Table policies: the workgroup's W waves run W trajectories of one layout and
-// share its table in LDS: the pair table (S*S + 192 float2) and, for BT, the
-// c table (N*S + 192 floats).
+// share its tables in LDS.  Layout: the waves' electrode factors (W*2*PMAX
+// floats), the electrode weights (elw_lds_floats), the pair table
+// (S*S + 64C float2) and, for BT, the c table (N*S + 64C floats).
 template <int C, int W, bool REC, bool BT>
 __global__ void __launch_bounds__(64 * W) kmc_batch_fp32_kernel(BatchParams p) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -18,7 +19,8 @@ __global__ void __launch_bounds__(64 * W) kmc_batch_fp32_kernel(BatchParams p) {
   // from them) in scalar registers with scalar branches
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
-  float2* tab = reinterpret_cast<float2*>(smem);
+  float* evc = reinterpret_cast<float*>(smem) + wave * 2 * PMAX;
+  float* elw = reinterpret_cast<float*>(smem) + W * 2 * PMAX;
   __shared__ int s_group;
   __shared__ int s_layout;
   if (threadIdx.x == 0) s_layout = -1;
@@ -30,13 +32,16 @@ __global__ void __launch_bounds__(64 * W) kmc_batch_fp32_kernel(BatchParams p) {
     if (g >= p.ngroups) break;
     const GroupDev gd = p.groups[g];
     const LayoutDev L = p.layouts[gd.layout];
-    float* ctab = reinterpret_cast<float*>(tab + L.S * L.S + 64 * 3);
+    float2* tab = reinterpret_cast<float2*>(elw + elw_lds_floats(L.P, L.S, C));
+    float* ctab = reinterpret_cast<float*>(tab + L.S * L.S + 64 * C);
     if (gd.layout != __builtin_amdgcn_readfirstlane(s_layout)) {
-      const int n = L.S * L.S + 64 * 3;
+      const int ne = ((L.P + 3) & ~3) * L.S + 64 * C;
+      for (int k = threadIdx.x; k < ne; k += 64 * W) elw[k] = L.elw[k];
+      const int n = L.S * L.S + 64 * C;
       const float2* src = BT ? L.tabB : L.tab32;
       for (int k = threadIdx.x; k < n; k += 64 * W) tab[k] = src[k];
       if (BT) {
-        const int nc = L.N * L.S + 64 * 3;
+        const int nc = L.N * L.S + 64 * C;
         for (int k = threadIdx.x; k < nc; k += 64 * W) ctab[k] = L.ccol[k];
       }
       __syncthreads();
@@ -44,7 +49,7 @@ __global__ void __launch_bounds__(64 * W) kmc_batch_fp32_kernel(BatchParams p) {
     }
     __syncthreads();
     if (wave < gd.count) {
-      const TabPolicy<C, BT> pol{tab, ctab, L.S, lane};
+      const TabPolicy<C, BT> pol{tab, ctab, elw, evc, L.S, lane};
       run_trajectory<C, REC>(p, gd.first + wave, L, pol, lane);
     }
   }
@@ -103,9 +108,13 @@ __global__ void kmc_eset_scatter_kernel(const EsetDev* esets, const int32_t* ids
   for (int k = threadIdx.x; k < P; k += blockDim.x) Vd[k] = V[i * PMAX + k];
 }
 
-size_t batch_lds_bytes(bool bt, int S, int N) {
-  size_t b = ((size_t)S * S + 64 * 3) * sizeof(float2);
-  if (bt) b += ((size_t)N * S + 64 * 3) * sizeof(float);
+// Upper bound of the LDS bytes of table-form layouts with at most S sites, N
+// acceptors and P electrodes, for W waves per workgroup.
+size_t batch_lds_bytes(bool bt, int C, int W, int S, int N, int P) {
+  size_t b = (size_t)W * 2 * PMAX * sizeof(float);
+  b += (size_t)elw_lds_floats(P, S, C) * sizeof(float);
+  b += ((size_t)S * S + 64 * C) * sizeof(float2);
+  if (bt) b += ((size_t)N * S + 64 * C) * sizeof(float);
   return b;
 }
 

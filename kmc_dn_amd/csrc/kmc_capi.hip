@@ -30,11 +30,11 @@ __global__ void kmc_queue_reset_kernel(int32_t*);
 __global__ void kmc_eset_scatter_kernel(const EsetDev*, const int32_t*, const int32_t*, const double*,
                                         const double*, int64_t);
 size_t plain_lds_bytes(int S, int N, bool stage);
-size_t batch_lds_bytes(bool bt, int S, int N);
+size_t batch_lds_bytes(bool bt, int C, int W, int S, int N, int P);
 size_t pos_lds_bytes(int C, int W);
 template <int C, int T, bool REC, int FORM>
 __global__ void kmc_team_kernel(BatchParams);
-size_t team_lds_bytes(int form, int C, int T, int S, int N);
+size_t team_lds_bytes(int form, int C, int T, int S, int N, int P);
 
 namespace {
 
@@ -97,7 +97,7 @@ struct Layout {
   ~Layout() {
     for (DevBuf* b : eset_bufs) delete b;
   }
-  DevBuf tab32, tabB, ccol, coul64, tc64, ird64, invd64, pos4;
+  DevBuf tab32, tabB, ccol, elw, coul64, tc64, ird64, invd64, pos4;
   LayoutDev dev{};
 };
 
@@ -107,7 +107,7 @@ struct Batch {
   int group_mode = -1;              // waves per group of the current groups (0: one per group)
   std::vector<TrajDev> host;        // in layout-grouped order
   std::vector<int64_t> perm;        // grouped position -> caller index
-  int smax = 0, nmax = 0;
+  int smax = 0, nmax = 0, pmax = 0;
   DevBuf trajs, out, occ_time, traffic, queue, groups;
   int32_t ngroups = 0;
   bool last_rec = false;
@@ -122,7 +122,7 @@ struct Batch {
   struct Sub {
     DevBuf groups;
     int32_t ngroups = 0;
-    int smax = 0, nmax = 0;
+    int smax = 0, nmax = 0, pmax = 0;
   };
   Sub subs[KMC_MAX_BUCKETS];
   bool subs_built = false;
@@ -240,6 +240,14 @@ int layout_create_locked(int64_t N, int64_t P, double nu, double kT, double I_0,
   for (auto& x : tabB) x = make_float2(-INFINITY, -INFINITY);
   std::vector<double> coul((size_t)n * n, 0.0), tc64(tc, tc + (size_t)S * S),
       ird((size_t)S * S, 0.0), invd((size_t)n * n, 0.0);
+  std::vector<float> elw((size_t)((P + 3) & ~3) * S + 64 * 3, 0.f);
+  bool elw_ok = true;
+  for (int e = 0; e < (int)P; ++e)
+    for (int s2 = 0; s2 < n; ++s2) {
+      const double t = tc[(size_t)s2 * S + n + e];
+      if (t > 0 && t > cut) elw[(size_t)e * S + s2] = (float)(nu * t);
+      elw_ok = elw_ok && std::isfinite(elw[(size_t)e * S + s2]);
+    }
   const double lnu = std::log2(nu);
   for (int u = 0; u < S; ++u)
     for (int s = 0; s < S; ++s) {
@@ -266,6 +274,7 @@ int layout_create_locked(int64_t N, int64_t P, double nu, double kT, double I_0,
   L->S = S;
   int st;
   if ((st = upload(L->tab32, tab)) || (st = upload(L->tabB, tabB)) || (st = upload(L->ccol, ccol)) ||
+      (st = upload(L->elw, elw)) ||
       (st = upload(L->coul64, coul)) || (st = upload(L->tc64, tc64)) ||
       (st = upload(L->ird64, ird)) || (st = upload(L->invd64, invd))) {
     delete L;
@@ -282,9 +291,11 @@ int layout_create_locked(int64_t N, int64_t P, double nu, double kT, double I_0,
   D.R = R;
   D.I0R = I0R;
   D.scale2 = (float)scale2;
+  D.elw_ok = elw_ok ? 1 : 0;
   D.tab32 = L->tab32.as<float2>();
   D.tabB = L->tabB.as<float2>();
   D.ccol = L->ccol.as<float>();
+  D.elw = L->elw.as<float>();
   D.coul64 = L->coul64.as<double>();
   D.tc64 = L->tc64.as<double>();
   D.ird64 = L->ird64.as<double>();
@@ -527,7 +538,7 @@ int batch_prepare(Batch* B, const kmc_traj* t, int64_t ntraj) {
     return t[a].prehops + t[a].hops > t[b].prehops + t[b].hops;
   });
   B->host.resize(ntraj);
-  B->smax = B->nmax = 0;
+  B->smax = B->nmax = B->pmax = 0;
   for (int64_t i = 0; i < ntraj; ++i) {
     const kmc_traj& x = t[B->perm[i]];
     TrajDev& d = B->host[i];
@@ -542,6 +553,7 @@ int batch_prepare(Batch* B, const kmc_traj* t, int64_t ntraj) {
     d.h_end = x.prehops + x.hops;
     B->smax = std::max(B->smax, E.layouts[x.layout]->S);
     B->nmax = std::max(B->nmax, E.layouts[x.layout]->N);
+    B->pmax = std::max(B->pmax, E.layouts[x.layout]->P);
   }
   B->group_mode = -1;
   B->subs_built = false;
@@ -717,7 +729,7 @@ struct Fp32Launch {
 // One work-queue kernel over `ngroups` groups.  form: 1 {lt, lt+c} + c table,
 // 2 {lt, c} table, 3 positions; smax/nmax bound the S and N of its layouts.
 int setup_launch(Batch* B, const kmc_run_opts& o, int form, int W, int team, int smax, int nmax,
-                 const DevBuf& groups, int32_t ngroups, int32_t* queue, int grid_cap,
+                 int pmax, const DevBuf& groups, int32_t ngroups, int32_t* queue, int grid_cap,
                  Fp32Launch& L) {
   Engine& E = eng();
   const int C = (smax + 63) / 64;
@@ -727,14 +739,14 @@ int setup_launch(Batch* B, const kmc_run_opts& o, int form, int W, int team, int
   size_t lds = 0;
   if (team > 1) {
     k = pick_team_kernel(C, team, rec, form);
-    lds = team_lds_bytes(form, C, team, smax, nmax);
+    lds = team_lds_bytes(form, C, team, smax, nmax, pmax);
     W = team;   // block of 64 * team threads, one trajectory per workgroup
   } else if (form == 3) {
     k = pick_pos_kernel(C, W, rec);
     lds = pos_lds_bytes(C, W);
   } else {
     k = pick_batch_kernel(C, W, rec, form == 1);
-    lds = batch_lds_bytes(form == 1, smax, nmax);
+    lds = batch_lds_bytes(form == 1, C, W, smax, nmax, pmax);
   }
   if (!k) return fail(KMC_ERR_LIMIT, "no kernel for %d site chunks", C);
   if (lds > lds_cap)
@@ -803,7 +815,7 @@ int build_bucket_groups(Batch* B) {
   Engine& E = eng();
   const int nb = (int)B->plan.size();
   std::vector<std::vector<GroupDev>> g(nb);
-  for (int b = 0; b < nb; ++b) B->subs[b].smax = B->subs[b].nmax = 0;
+  for (int b = 0; b < nb; ++b) B->subs[b].smax = B->subs[b].nmax = B->subs[b].pmax = 0;
   auto bucket_at = [&](int64_t k) {
     const TrajDev& t = B->host[k];
     return bucket_of(B, E.layouts[t.layout]->S, t.prehops + t.hops);
@@ -829,6 +841,7 @@ int build_bucket_groups(Batch* B) {
     g[b].push_back(gd);
     B->subs[b].smax = std::max(B->subs[b].smax, Ly->S);
     B->subs[b].nmax = std::max(B->subs[b].nmax, Ly->N);
+    B->subs[b].pmax = std::max(B->subs[b].pmax, Ly->P);
     i = j;
   }
   for (int b = 0; b < nb; ++b) {
@@ -860,7 +873,7 @@ int prepare_fp32(Batch* B, const kmc_run_opts& o, std::vector<Fp32Launch>& out) 
       if (pb.table_form == 3 && (st = check_positions(B, b))) return st;
       Fp32Launch L;
       st = setup_launch(B, o, pb.table_form, pb.waves_per_group, pb.team, sb.smax, sb.nmax,
-                        sb.groups, sb.ngroups, q + 16 * b, pb.grid > 0 ? pb.grid : o.grid_limit, L);
+                        sb.pmax, sb.groups, sb.ngroups, q + 16 * b, pb.grid > 0 ? pb.grid : o.grid_limit, L);
       if (st) return st;
       out.push_back(L);
     }
@@ -868,10 +881,11 @@ int prepare_fp32(Batch* B, const kmc_run_opts& o, std::vector<Fp32Launch>& out) 
   }
   const int C = (B->smax + 63) / 64;
   int form = o.table_form;
-  if (form == 0) form = batch_lds_bytes(true, B->smax, B->nmax) <= (size_t)160 * 1024 ? 1 : 2;
-  if (form < 1 || form > 3) return fail(KMC_ERR_ARGS, "table_form must be 0, 1, 2 or 3");
   int W = o.waves_per_group;
   if (W <= 0) W = form == 3 ? 4 : (C == 1 ? 4 : (C == 2 ? 2 : 1));
+  if (form == 0)
+    form = batch_lds_bytes(true, C, W, B->smax, B->nmax, B->pmax) <= (size_t)160 * 1024 ? 1 : 2;
+  if (form < 1 || form > 3) return fail(KMC_ERR_ARGS, "table_form must be 0, 1, 2 or 3");
   if (W != 1 && W != 2 && W != 4 && W != 8) return fail(KMC_ERR_ARGS, "waves_per_group must be 1, 2, 4 or 8");
   const int team = o.team <= 1 ? 1 : o.team;
   if (team != 1 && team != 2 && team != 4) return fail(KMC_ERR_ARGS, "team must be 1, 2 or 4");
@@ -879,7 +893,7 @@ int prepare_fp32(Batch* B, const kmc_run_opts& o, std::vector<Fp32Launch>& out) 
   // positions and teams: one trajectory per group
   if ((st = build_groups(B, (form == 3 || team > 1) ? 1 : W))) return st;
   Fp32Launch L;
-  if ((st = setup_launch(B, o, form, W, team, B->smax, B->nmax, B->groups, B->ngroups, q,
+  if ((st = setup_launch(B, o, form, W, team, B->smax, B->nmax, B->pmax, B->groups, B->ngroups, q,
                          o.grid_limit, L)))
     return st;
   out.push_back(L);

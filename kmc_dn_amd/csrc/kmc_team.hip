@@ -39,22 +39,28 @@ __global__ void __launch_bounds__(64 * T) kmc_team_kernel(BatchParams p) {
       pol.init(pos, L, lane);
       for (int i = 0; i < gd.count; ++i) run_trajectory<C, REC>(p, gd.first + i, L, pol, lane, team);
     } else {
+      // the waves' electrode factors (T*2*PMAX floats), the electrode weights,
+      // the pair table and, for FORM 1, the c table
       constexpr bool BT = FORM == 1;
-      float2* tab = reinterpret_cast<float2*>(rest);
-      float* ctab = reinterpret_cast<float*>(tab + L.S * L.S + 64 * 3);
+      float* evc = reinterpret_cast<float*>(rest) + wave * 2 * PMAX;
+      float* elw = reinterpret_cast<float*>(rest) + T * 2 * PMAX;
+      float2* tab = reinterpret_cast<float2*>(elw + elw_lds_floats(L.P, L.S, C));
+      float* ctab = reinterpret_cast<float*>(tab + L.S * L.S + 64 * C);
       if (gd.layout != __builtin_amdgcn_readfirstlane(s_layout)) {
-        const int n = L.S * L.S + 64 * 3;
+        const int ne = ((L.P + 3) & ~3) * L.S + 64 * C;
+        for (int k = threadIdx.x; k < ne; k += 64 * T) elw[k] = L.elw[k];
+        const int n = L.S * L.S + 64 * C;
         const float2* src = BT ? L.tabB : L.tab32;
         for (int k = threadIdx.x; k < n; k += 64 * T) tab[k] = src[k];
         if (BT) {
-          const int nc = L.N * L.S + 64 * 3;
+          const int nc = L.N * L.S + 64 * C;
           for (int k = threadIdx.x; k < nc; k += 64 * T) ctab[k] = L.ccol[k];
         }
         __syncthreads();
         if (threadIdx.x == 0) s_layout = gd.layout;
       }
       __syncthreads();
-      const TabPolicy<C, BT> pol{tab, ctab, L.S, lane};
+      const TabPolicy<C, BT> pol{tab, ctab, elw, evc, L.S, lane};
       for (int i = 0; i < gd.count; ++i) run_trajectory<C, REC>(p, gd.first + i, L, pol, lane, team);
     }
   }
@@ -70,12 +76,15 @@ __global__ void __launch_bounds__(64 * T) kmc_team_kernel(BatchParams p) {
 KMC_TEAM(1, 2) KMC_TEAM(1, 4) KMC_TEAM(2, 2) KMC_TEAM(2, 4) KMC_TEAM(3, 2) KMC_TEAM(3, 4)
 #undef KMC_TEAM
 
-// LDS bytes of a team workgroup: partial row sums plus the form's table or positions.
-size_t team_lds_bytes(int form, int C, int T, int S, int N) {
+// LDS bytes of a team workgroup: partial row sums plus the form's tables or
+// positions (an upper bound for layouts with at most S, N and P).
+size_t team_lds_bytes(int form, int C, int T, int S, int N, int P) {
   size_t b = (size_t)2 * T * C * 64 * sizeof(float);
   if (form == 3) return b + (size_t)64 * C * sizeof(float4);
-  b += ((size_t)S * S + 64 * 3) * sizeof(float2);
-  if (form == 1) b += ((size_t)N * S + 64 * 3) * sizeof(float);
+  b += (size_t)T * 2 * PMAX * sizeof(float);
+  b += (size_t)elw_lds_floats(P, S, C) * sizeof(float);
+  b += ((size_t)S * S + 64 * C) * sizeof(float2);
+  if (form == 1) b += ((size_t)N * S + 64 * C) * sizeof(float);
   return b;
 }
 

@@ -20,6 +20,12 @@
 //    {targets} (empty acceptors + electrodes) is walked four members at a time
 //    with a scalar loop while the lanes cover every site of the other role:
 //    row sums R_lane accumulate in registers with no cross-lane traffic.
+//  * Table policies leave the electrodes out of the walked set: with
+//    w = nu * tc[s][e] the rates between acceptor s and all electrodes sum to
+//        sum_e w * min(1, 2^e~_s * 2^-V~_e)   (s occupied, hops into them) or
+//        sum_e w * min(1, 2^-e~_s * 2^V~_e)   (s empty, hops out of them),
+//    one product, min and FMA per electrode and no exp2 per pair
+//    (elec_sums).  The per-trajectory factors 2^+-V~_e sit in LDS.
 //  * Event pick: FP64 wave prefix sum of R (DPP), Philox event uniform * total,
 //    ballot for the first lane whose prefix exceeds it, then one lane-parallel
 //    pass over that lane's pairs (FP32 DPP scan) for the partner site.
@@ -44,8 +50,15 @@
 
 namespace kmc {
 
+// LDS floats of a layout's electrode weights (P rounded up to 4 rows of S, the
+// padding rows zero, then 64C zeros), rounded up to 16 bytes so the pair table
+// behind them stays aligned.
+__host__ __device__ inline int elw_lds_floats(int P, int S, int C) {
+  return (((P + 3) & ~3) * S + 64 * C + 3) & ~3;
+}
+
 template <int C>
-__device__ __forceinline__ void recompute_energies(float (&se)[C], const uint64_t (&occ)[OCCW],
+__device__ __forceinline__ void recompute_energies(float (&se)[C], const uint64_t (&occ)[C],
                                                    const LayoutDev& L, const EsetDev& es,
                                                    int lane) {
   const int N = L.N, S = L.S;
@@ -108,8 +121,11 @@ __device__ __forceinline__ float min_f32(float a, float b) {
 // ---------------------------------------------------------------------------
 template <int C, bool BT>
 struct TabPolicy {
-  const float2* tab;   // [S*S + 192]: entry (s, u) at u*S + s; column S is all -inf
-  const float* ctab;   // BT: [N*S + 192] c at u*S + s
+  static constexpr bool kElec = true;   // electrodes in closed form (elec_sums)
+  const float2* tab;   // [S*S + 64C]: entry (s, u) at u*S + s; column S is all -inf
+  const float* ctab;   // BT: [N*S + 64C] c at u*S + s
+  const float* elw;    // [P*S + 64C] electrode weights, e*S + s
+  float* evc;          // this wave's [2][PMAX]: 2^V~_e, then 2^-V~_e
   int S, lane;
   struct Pre {
     float2 e[C];
@@ -134,6 +150,9 @@ struct TabPolicy {
 
 template <int C>
 struct PosPolicy {
+  static constexpr bool kElec = false;   // electrodes are walked as members
+  const float* elw = nullptr;
+  float* evc = nullptr;
   const float4* pos;   // [S] per wave: x, y, z, 1 for acceptors / 0 for electrodes
   float px[C], py[C], pz[C], kcl[C];   // the lane sites' coordinates and c scale
   uint64_t elm[C];     // lanes of chunk c that hold electrodes
@@ -265,6 +284,41 @@ __device__ __forceinline__ void row_sums(const Pol& pol, int S, const uint64_t (
   }
 }
 
+// Closed-form electrode terms of the row sums (table policies).  v: the lane
+// sites' scaled energies in the lane role, -inf (udst) / +inf elsewhere, so
+// 2^(+-v) is 0 on lanes outside the role; electrode lanes have w = 0.  The
+// exponent is clamped to 126 (then x -+ V~ >= 26 and the min is 1 either way);
+// |V~| <= 100 keeps 2^-+V~ finite and nonzero, so the product is never NaN.
+// Electrodes go in blocks of four (all loads of a block issued together); rows
+// past P have w = 0 and factor 1.  A team splits the blocks: wave w takes the
+// blocks j = w (mod T).
+template <int C, int T, class Pol>
+__device__ __forceinline__ void elec_sums(const Pol& pol, int P, int S, bool udst, int wave,
+                                          const float (&v)[C], float (&R)[C]) {
+  float ex[C];
+#pragma unroll
+  for (int c = 0; c < C; ++c) ex[c] = fast_exp2(min_f32(udst ? v[c] : -v[c], 126.f));
+  const float* k = pol.evc + (udst ? PMAX : 0);
+  const float* w = pol.elw + pol.lane;
+  const int nb = (P + 3) >> 2;
+  for (int j = wave; j < nb; j += T) {
+    const float4 k4 = *reinterpret_cast<const float4*>(k + 4 * j);
+    const float kq[4] = {k4.x, k4.y, k4.z, k4.w};
+    float wq[4][C];
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+      for (int c = 0; c < C; ++c) wq[q][c] = w[(4 * j + q) * S + 64 * c];
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+      for (int c = 0; c < C; ++c) {
+        if (c > 0 && 64 * c >= S) break;
+        R[c] = fmaf(wq[q][c], min_f32(1.f, ex[c] * kq[q]), R[c]);
+      }
+  }
+}
+
 template <int C>
 __device__ __forceinline__ float pick_lane_value(const float (&x)[C], int site) {
   const int cs = site >> 6, ls = site & 63;
@@ -288,12 +342,12 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
   const int64_t h_end = p.resume ? (td.h_end < total_hops ? td.h_end : total_hops) : total_hops;
   const bool cont = p.resume && h_begin > 0;
 
-  uint64_t occ[OCCW];
+  uint64_t occ[C];   // acceptors (N <= 64C) live in the first C words
   float se[C];
   int eo[C];
   double ot[C];
 #pragma unroll
-  for (int w = 0; w < OCCW; ++w) occ[w] = cont ? o->occ[w] : td.occ[w];
+  for (int w = 0; w < C; ++w) occ[w] = cont ? o->occ[w] : td.occ[w];
 #pragma unroll
   for (int c = 0; c < C; ++c) {
     const int s = lane + 64 * c;
@@ -309,6 +363,21 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
     accm[c] = chunk_mask(0, N, c);
     elm[c] = chunk_mask(N, S, c);
   }
+  // electrodes in closed form when the policy has the weights and every scaled
+  // voltage satisfies |V~| <= 100; otherwise they are walked as members
+  bool elec = false;
+  if constexpr (Pol::kElec) {
+    const float vt = lane < L.P ? (float)(es.V[lane] * (double)L.scale2) : 0.f;
+    elec = L.P > 0 && L.elw_ok && ballot(!(fabsf(vt) <= 100.f)) == 0;
+    if (lane < PMAX) {   // 2^0 = 1 past P: finite factors for the zero padding rows
+      pol.evc[lane] = fast_exp2(vt);
+      pol.evc[PMAX + lane] = fast_exp2(-vt);
+    }
+    wave_lds_sync();
+  }
+  uint64_t walk[C];   // sites that may be walked: acceptors, plus electrodes unless elec
+#pragma unroll
+  for (int c = 0; c < C; ++c) walk[c] = elec ? accm[c] : ~0ull;
 
   const int K = p.recompute_interval;
   float lg_lane = 0.f;    // log2(1 - u_clock) of hop (h & ~63) + lane
@@ -349,13 +418,18 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
     const bool udst = ndst <= nsrc;   // walk the smaller set
     const float vfill = udst ? -INFINITY : INFINITY;
     float v[C], R[C];
+    uint64_t um[C];
 #pragma unroll
     for (int c = 0; c < C; ++c) {
       R[c] = 0.f;
       v[c] = __builtin_amdgcn_inverse_ballot_w64(udst ? srcm[c] : dstm[c]) ? se[c] : vfill;
+      um[c] = (udst ? dstm[c] : srcm[c]) & walk[c];
     }
-    if (udst) row_sums<C, true>(pol, S, dstm, team.own(), v, se, R);
-    else row_sums<C, false>(pol, S, srcm, team.own(), v, se, R);
+    if constexpr (Pol::kElec) {
+      if (elec) elec_sums<C, T>(pol, L.P, S, udst, team.wave, v, R);
+    }
+    if (udst) row_sums<C, true>(pol, S, um, team.own(), v, se, R);
+    else row_sums<C, false>(pol, S, um, team.own(), v, se, R);
     team.reduce(R, (int)(h & 1), lane);
 
     // FP64 prefix over lane rows
@@ -481,7 +555,7 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
     o->hops_done = h > td.prehops ? h - td.prehops : 0;
     o->status = status;
 #pragma unroll
-    for (int w = 0; w < OCCW; ++w) o->occ[w] = occ[w];
+    for (int w = 0; w < OCCW; ++w) o->occ[w] = w < C ? occ[w] : 0ull;
   }
 #pragma unroll
   for (int c = 0; c < C; ++c) {

@@ -23,7 +23,7 @@ struct LayoutDev {
   double nu, kT, I0, R;
   double I0R;        // I_0 * R
   float scale2;      // log2(e) / kT: energies in kT units, base-2 exponent
-  float pad0;
+  int32_t elw_ok;    // every electrode weight is finite (elw usable)
   // FP32 batched kernel.  tab32[u * S + s] = {lt, c} for the ordered pair
   // (s, u), symmetric in (s, u):
   //   lt = log2(nu * transitions_constant[s][u])  (-inf when impossible:
@@ -35,6 +35,11 @@ struct LayoutDev {
   // table ccol[u * S + s] = c(s, u) for acceptor columns u < N (N*S + 192 floats).
   const float2* tabB;
   const float* ccol;
+  // Electrode weights elw[e * S + s] = nu * tc[s][N + e] for acceptor sites s
+  // (0 for electrode sites and pruned pairs; rows rounded up to a multiple of 4,
+  // then 192 floats, the padding zero): the
+  // rate between acceptor s and electrode e is elw * min(1, 2^(e~_src - e~_dst)).
+  const float* elw;
   const double* coul64;   // [N*N] I_0*R/d for acceptor pairs (symmetric, 0 diagonal)
   // Positions policy: site coordinates (w = 1 acceptor, 0 electrode) and the
   // closed forms lt = pos_l0 + pos_mkl * d, c = pos_kc / d (null if absent).

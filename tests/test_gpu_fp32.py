@@ -295,3 +295,50 @@ def test_pruned_pairs_never_hop(gpu_engine):
         assert moved.any() and not (moved & pruned).any()
     b.close()
     lay.close()
+
+
+def test_electrode_closed_form_and_member_fallback(gpu_engine):
+    """Table forms add the electrode rates in closed form while every scaled
+    voltage |V log2(e) / kT| <= 100 and walk the electrodes as members otherwise;
+    the positions form always walks them.  One batch mixes both cases: each
+    agrees with the positions form statistically, and the mix changes no bits."""
+    net = random_network(23, 40, 8, I_0=30.0, a=0.2, vmax=2.0)
+    N, P = 40, 8
+    lay = engine.Layout(N, P, 1.0, 1.0, net["I_0"], net["R"], net["distances"],
+                        net["transitions_constant"])
+    lay.set_positions(0.2 * net["R"], _positions(net))
+    v_small = net["site_energies"][N:]
+    v_large = np.where(np.arange(P) % 2 == 0, 75.0, -75.0)     # |V~| = 108
+    es = lay.energy_sets(np.stack([net["E_constant"]] * 2), np.stack([v_small, v_large]))
+    K, hops = 512, 5000
+    trajs = np.concatenate([
+        engine.make_trajs(K, lay.id, es[0], np.arange(K, dtype=np.uint64), hops,
+                          net["occupation"], prehops=500),
+        engine.make_trajs(K, lay.id, es[1], np.arange(K, 2 * K, dtype=np.uint64), hops,
+                          net["occupation"], prehops=500)])
+    b = engine.Batch(trajs)
+    out = {}
+    try:
+        for form in (1, 2, 3):
+            print(f"[gpu] electrode closed form / fallback, form {form}", flush=True)
+            b.run(precision=_lib.FP32_BATCHED, table_form=form)
+            b.sync()
+            r = b.results()
+            assert (r.status == 0).all() and (r.hops_done == hops).all()
+            out[form] = r
+    finally:
+        b.close()
+    alone = engine.Batch(trajs[K:])
+    try:
+        alone.run(precision=_lib.FP32_BATCHED, table_form=1)
+        alone.sync()
+        a = alone.results()
+    finally:
+        alone.close()
+    assert np.array_equal(a.eo, out[1].eo[K:]) and np.array_equal(a.time, out[1].time[K:])
+    assert np.array_equal(a.occ_bits, out[1].occ_bits[K:])
+    lay.close()
+    for half, sl in (("closed form", slice(0, K)), ("fallback", slice(K, 2 * K))):
+        pos = out[3].currents(P)[sl]
+        for form in (1, 2):
+            _compare(out[form].currents(P)[sl], pos, f"{half}: table {form} vs positions")
