@@ -40,7 +40,7 @@ LDS_PEAK_SPEC_GBS = 256 * 256 * 2.4
 # N=30/P=8, auto table form and workgroup size): rocprofv3 --pmc FETCH_SIZE 379.125 KiB
 # and WRITE_SIZE 640 KiB per dispatch, scaled by the copy-kernel calibration factors
 # 2.0 (FETCH) and 1.0 (WRITE): (2 * 379.125 + 640) * 1024 bytes.
-HBM_TRAFFIC_BYTES = 1_431_808
+HBM_TRAFFIC_BYTES = 1_424_384
 
 
 def lds_bytes_per_hop(N: int, P: int) -> int:

@@ -240,13 +240,13 @@ int layout_create_locked(int64_t N, int64_t P, double nu, double kT, double I_0,
   for (auto& x : tabB) x = make_float2(-INFINITY, -INFINITY);
   std::vector<double> coul((size_t)n * n, 0.0), tc64(tc, tc + (size_t)S * S),
       ird((size_t)S * S, 0.0), invd((size_t)n * n, 0.0);
-  std::vector<float> elw((size_t)((P + 3) & ~3) * S + 64 * 3, 0.f);
+  std::vector<float> elw((size_t)((P + 3) & ~3) * 192, 0.f);
   bool elw_ok = true;
   for (int e = 0; e < (int)P; ++e)
     for (int s2 = 0; s2 < n; ++s2) {
       const double t = tc[(size_t)s2 * S + n + e];
-      if (t > 0 && t > cut) elw[(size_t)e * S + s2] = (float)(nu * t);
-      elw_ok = elw_ok && std::isfinite(elw[(size_t)e * S + s2]);
+      if (t > 0 && t > cut) elw[(size_t)e * 192 + s2] = (float)(nu * t);
+      elw_ok = elw_ok && std::isfinite(elw[(size_t)e * 192 + s2]);
     }
   const double lnu = std::log2(nu);
   for (int u = 0; u < S; ++u)

@@ -50,11 +50,16 @@
 
 namespace kmc {
 
-// LDS floats of a layout's electrode weights (P rounded up to 4 rows of S, the
-// padding rows zero, then 64C zeros), rounded up to 16 bytes so the pair table
-// behind them stays aligned.
-__host__ __device__ inline int elw_lds_floats(int P, int S, int C) {
-  return (((P + 3) & ~3) * S + 64 * C + 3) & ~3;
+// LDS floats of a layout's electrode weights: P rounded up to 4 rows (padding
+// rows zero) of 64C lane sites, so every load of a block of four electrodes has
+// an immediate offset.
+__host__ __device__ inline int elw_lds_floats(int P, int C) { return ((P + 3) & ~3) * 64 * C; }
+
+// Copies the electrode weights (global rows of 192 sites) into LDS rows of 64C.
+template <int C>
+__device__ __forceinline__ void load_elw(float* dst, const LayoutDev& L, int tid, int nthreads) {
+  const int n = elw_lds_floats(L.P, C);
+  for (int k = tid; k < n; k += nthreads) dst[k] = L.elw[(k / (64 * C)) * 192 + k % (64 * C)];
 }
 
 template <int C>
@@ -84,6 +89,13 @@ __device__ __forceinline__ void recompute_energies(float (&se)[C], const uint64_
 __device__ __forceinline__ int lowest_bit(uint64_t m) {
   int r;
   asm("s_ff1_i32_b64 %0, %1" : "=s"(r) : "s"(m));
+  return r;
+}
+
+// Set bits of a wave-uniform mask as a 32-bit scalar (s_bcnt1 sets SCC).
+__device__ __forceinline__ int popcount64(uint64_t m) {
+  int r;
+  asm("s_bcnt1_i32_b64 %0, %1" : "=s"(r) : "s"(m) : "scc");
   return r;
 }
 
@@ -124,7 +136,7 @@ struct TabPolicy {
   static constexpr bool kElec = true;   // electrodes in closed form (elec_sums)
   const float2* tab;   // [S*S + 64C]: entry (s, u) at u*S + s; column S is all -inf
   const float* ctab;   // BT: [N*S + 64C] c at u*S + s
-  const float* elw;    // [P*S + 64C] electrode weights, e*S + s
+  const float* elw;    // [4*ceil(P/4)][64C] electrode weights of the lane sites
   float* evc;          // this wave's [2][PMAX]: 2^V~_e, then 2^-V~_e
   int S, lane;
   struct Pre {
@@ -308,7 +320,7 @@ __device__ __forceinline__ void elec_sums(const Pol& pol, int P, int S, bool uds
 #pragma unroll
     for (int q = 0; q < 4; ++q)
 #pragma unroll
-      for (int c = 0; c < C; ++c) wq[q][c] = w[(4 * j + q) * S + 64 * c];
+      for (int c = 0; c < C; ++c) wq[q][c] = w[(4 * j + q) * 64 * C + 64 * c];
 #pragma unroll
     for (int q = 0; q < 4; ++q)
 #pragma unroll
@@ -337,9 +349,12 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
   const EsetDev es = p.esets[td.eset];
   const int N = L.N, S = L.S;
   TrajOut* o = p.out + t;
-  const int64_t total_hops = td.prehops + td.hops;
-  const int64_t h_begin = p.resume ? td.h_begin : 0;
-  const int64_t h_end = p.resume ? (td.h_end < total_hops ? td.h_end : total_hops) : total_hops;
+  // hop counters are 32-bit (prehops + hops <= INT32_MAX is checked when a batch
+  // is created), so their comparisons stay on the scalar unit
+  const int64_t total64 = td.prehops + td.hops;
+  const int64_t hb64 = p.resume ? (td.h_begin < total64 ? td.h_begin : total64) : 0;
+  const int64_t he64 = p.resume ? (td.h_end < total64 ? td.h_end : total64) : total64;
+  const int h_begin = (int)hb64, h_end = (int)he64, prehops = (int)td.prehops;
   const bool cont = p.resume && h_begin > 0;
 
   uint64_t occ[C];   // acceptors (N <= 64C) live in the first C words
@@ -355,7 +370,7 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
     ot[c] = (cont && REC && p.occ_time && s < N) ? p.occ_time[(size_t)t * NMAX + s] : 0.0;
   }
   double time = cont ? o->time : 0.0;
-  int32_t status = cont ? o->status : (int32_t)ST_OK;
+  int32_t status = __builtin_amdgcn_readfirstlane(cont ? o->status : (int32_t)ST_OK);
   recompute_energies<C>(se, occ, L, es, lane);
   uint64_t accm[C], elm[C];   // lanes of chunk c holding acceptors / electrodes
 #pragma unroll
@@ -382,16 +397,21 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
   const int K = p.recompute_interval;
   float lg_lane = 0.f;    // log2(1 - u_clock) of hop (h & ~63) + lane
   double ue_lane = 0.0;   // event uniform of that hop
-  int64_t next_rc = (h_begin / K + 1) * K;   // next FP64 recompute (multiple of K)
-  const int64_t h_stop = status == ST_OK ? h_end : h_begin;   // a stopped trajectory stays
-  int64_t h = h_begin;
+  // next FP64 recompute (a multiple of K; INT32_MAX: none left in this range)
+  auto next_multiple = [K](int64_t x) {
+    const int64_t n = (x / K + 1) * K;
+    return (int)(n < INT32_MAX ? n : INT32_MAX);
+  };
+  int next_rc = next_multiple(h_begin);
+  const int h_stop = status == ST_OK ? h_end : h_begin;   // a stopped trajectory stays
+  int h = h_begin;
   for (; h < h_stop; ++h) {
     if (h == h_begin || (h & 63) == 0) {
       double uc;
       stream_uniforms(td.seed, td.hop0 + (uint64_t)((h & ~(int64_t)63) + lane), uc, ue_lane);
       lg_lane = fast_log2((float)(1.0 - uc));
     }
-    if (__builtin_expect(h == td.prehops, 0)) {
+    if (__builtin_expect(h == prehops, 0)) {
       time = 0.0;
 #pragma unroll
       for (int c = 0; c < C; ++c) {
@@ -401,7 +421,7 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
     }
     if (h == next_rc) {
       recompute_energies<C>(se, occ, L, es, lane);
-      next_rc += K;
+      next_rc = next_multiple(h);
     }
 
     // roles of the lanes' sites, as wave-uniform lane masks (scalar ops only):
@@ -412,8 +432,8 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
     for (int c = 0; c < C; ++c) {
       srcm[c] = (occ[c] & accm[c]) | elm[c];
       dstm[c] = (~occ[c] & accm[c]) | elm[c];
-      nsrc += __builtin_popcountll(srcm[c]);
-      ndst += __builtin_popcountll(dstm[c]);
+      nsrc += popcount64(srcm[c]);
+      ndst += popcount64(dstm[c]);
     }
     const bool udst = ndst <= nsrc;   // walk the smaller set
     const float vfill = udst ? -INFINITY : INFINITY;
@@ -480,7 +500,8 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
 
     // partner of `sel`: lane-parallel pass over its row, in site order
     const float s_sel = pick_lane_value<C>(se, sel);
-    const float sg = udst ? -1.f : 1.f;
+    // sign of the lane-parallel difference: a scalar constant
+    const float sg = __int_as_float(udst ? (int)0xbf800000 : (int)0x3f800000);
     const typename Pol::Pre pre = pol.prefetch(sel);
     float run = 0.f;
     int pick = -1, last_nz = -1;
@@ -488,8 +509,8 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
     for (int c = 0; c < C; ++c) {
       if (c > 0 && 64 * c >= S) break;
       const bool inU = __builtin_amdgcn_inverse_ballot_w64(udst ? dstm[c] : srcm[c]);
-      // udst ? s_sel - se : se - s_sel, one FMA (the sign is exact)
-      const float diff = fmaf(sg, se[c], -sg * s_sel);
+      // udst ? s_sel - se : se - s_sel (the sign is exact)
+      const float diff = sg * (se[c] - s_sel);
       float r = fast_exp2(pol.pair(pre, c, diff));
       r = inU ? r : 0.f;
       const float cum = wave_scan_f32(r) + run;
@@ -508,7 +529,7 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
     }
     const int src = udst ? sel : pick;
     const int dst = udst ? pick : sel;
-    const bool tally = h >= td.prehops;
+    const bool tally = h >= prehops;
 
     if (REC && tally && !p.post_hop_record) {
 #pragma unroll
@@ -552,7 +573,7 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
     const int64_t dt_ticks = wall_clock64() - t_start;
     o->ticks = (int32_t)(dt_ticks < INT32_MAX ? dt_ticks : INT32_MAX);
     o->time = time;
-    o->hops_done = h > td.prehops ? h - td.prehops : 0;
+    o->hops_done = h > prehops ? h - prehops : 0;
     o->status = status;
 #pragma unroll
     for (int w = 0; w < OCCW; ++w) o->occ[w] = w < C ? occ[w] : 0ull;

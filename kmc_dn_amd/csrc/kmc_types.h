@@ -35,9 +35,9 @@ struct LayoutDev {
   // table ccol[u * S + s] = c(s, u) for acceptor columns u < N (N*S + 192 floats).
   const float2* tabB;
   const float* ccol;
-  // Electrode weights elw[e * S + s] = nu * tc[s][N + e] for acceptor sites s
-  // (0 for electrode sites and pruned pairs; rows rounded up to a multiple of 4,
-  // then 192 floats, the padding zero): the
+  // Electrode weights elw[e * 192 + s] = nu * tc[s][N + e] for acceptor sites s
+  // (0 for electrode sites, s >= N and pruned pairs; rows rounded up to a
+  // multiple of 4, the padding zero): the
   // rate between acceptor s and electrode e is elw * min(1, 2^(e~_src - e~_dst)).
   const float* elw;
   const double* coul64;   // [N*N] I_0*R/d for acceptor pairs (symmetric, 0 diagonal)
