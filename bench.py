@@ -305,7 +305,7 @@ def main() -> int:
             "kernel_ms": float(np.mean(kernel_ms)),
             "note": "algorithmic LDS bytes of the dense formulation / kernel time; peak = "
                     "ds_read_b128 stream measured in this run; the kernel is bound by VALU "
-                    "issue (VALUBusy 72%, LdsUtil 9%)",
+                    "issue (VALUBusy 81%, LdsUtil 17%)",
         },
         "plain_fp64": {"value": plain_value, "unit": "hops/s",
                        "kernel_ms": float(np.mean(pms)),
