@@ -155,6 +155,20 @@ def uneven_batch(seed: int = 0, n: int = 2048, hop_range=(1e4, 1e6), N_range=(10
     return UnevenBatch(nets, hops, np.arange(n, dtype=np.uint64))
 
 
+def network_3d(seed: int, N: int, P: int, zdim: float = 0.2, I_0: float = 50.0, a: float = 0.1,
+               res: float = 0.01) -> Network:
+    """One network built as the uneven batch builds its networks, with a given N
+    and P (M = round(N / 10) donors)."""
+    rng = np.random.RandomState(seed)
+    M = int(round(N / 10))
+    acc = rng.rand(N, 3) * [1.0, 1.0, zdim]
+    don = rng.rand(M, 3) * [1.0, 1.0, zdim]
+    el = perimeter_points(P, zdim / 2)
+    el[:, 3] = rng.uniform(-1, 1, P)
+    eV = laplace.potential_3d_slab(acc, el, 1.0, 1.0, res=res)
+    return _finish(N, M, P, I_0, a, acc, don, el, (1.0, 1.0, zdim), rng, eV)
+
+
 @dataclass
 class Request:
     rid: int
