@@ -37,10 +37,16 @@ METRIC = "hops/sec, 4096-traj N=30 ensemble vs plain; p50/p99 latency of served 
 # divides by the LDS read peak measured in the same run (ds_read_b128 stream probe).
 LDS_PEAK_SPEC_GBS = 256 * 256 * 2.4
 # HBM bytes per launch of the batched kernel on the default workload (4096 x 1e5 hops,
-# N=30/P=8, auto table form and workgroup size): rocprofv3 --pmc FETCH_SIZE 379.125 KiB
+# N=30/P=8, auto table form and workgroup size): rocprofv3 --pmc FETCH_SIZE 375.5 KiB
 # and WRITE_SIZE 640 KiB per dispatch, scaled by the copy-kernel calibration factors
-# 2.0 (FETCH) and 1.0 (WRITE): (2 * 379.125 + 640) * 1024 bytes.
+# 2.0 (FETCH) and 1.0 (WRITE): (2 * 375.5 + 640) * 1024 bytes.
 HBM_TRAFFIC_BYTES = 1_424_384
+# VALU instructions per hop and wave of the batched kernel on the default workload
+# (rocprofv3 --pmc SQ_INSTS_VALU: 7.106e10 per dispatch of 4.096e8 hops), and the VALU
+# issue peak: one wave64 instruction per 4 cycles on each of a CU's 4 SIMDs (16 lanes),
+# i.e. one per CU and cycle, 256 CUs at 2.4 GHz.
+VALU_INSTS_PER_HOP = 173.5
+VALU_ISSUE_PEAK = 256 * 2.4e9
 
 
 def lds_bytes_per_hop(N: int, P: int) -> int:
@@ -303,9 +309,12 @@ def main() -> int:
             "bytes_per_hop": lds_bytes_per_hop(net.N, net.P),
             "hops_per_launch": hops_per_step,
             "kernel_ms": float(np.mean(kernel_ms)),
+            "valu_issue_frac": (VALU_INSTS_PER_HOP * hops_per_step / avg_kernel_s / VALU_ISSUE_PEAK
+                                if default_workload else None),
             "note": "algorithmic LDS bytes of the dense formulation / kernel time; peak = "
                     "ds_read_b128 stream measured in this run; the kernel is bound by VALU "
-                    "issue (VALUBusy 81%, LdsUtil 17%)",
+                    "issue (VALUBusy 81%, LdsUtil 17%): valu_issue_frac = profiled VALU "
+                    "instructions per hop x hops / kernel time / (1 per CU and cycle)",
         },
         "plain_fp64": {"value": plain_value, "unit": "hops/s",
                        "kernel_ms": float(np.mean(pms)),
