@@ -173,13 +173,14 @@ class Server:
 
     # ---- set-up -------------------------------------------------------------
     def _tune(self, log):
-        """Launch plan timed on at most 8 trajectories per layout (up to 1024): solo
-        buckets tuned for the throughput of a batch as sparse per layout as a
-        round's (groups of one layout are then partly filled, which favours more
-        waves per workgroup where a table limits workgroups per CU); team buckets
-        (budgets >= team_min_hops) for hop latency.  A request's bucket depends
-        only on its layout and budget."""
-        reps = max(1, min(8, 1024 // len(self.nets)))
+        """Launch plan timed on up to 32 trajectories per layout (at most 2048): solo
+        buckets tuned for the throughput of a batch larger than a loaded round, so
+        that a candidate whose workgroups per CU are limited by its table's LDS
+        pays for the slots it lacks (rounds under load hold more trajectories than
+        such a kernel has resident slots); team buckets (budgets >=
+        team_min_hops) for hop latency.  A request's bucket depends only on its
+        layout and budget."""
+        reps = max(1, min(32, 2048 // len(self.nets)))
         t = np.zeros(len(self.nets) * reps, dtype=engine.TRAJ_DTYPE)
         taken = []
         for i, net in enumerate(self.nets):
