@@ -134,6 +134,9 @@ __device__ __forceinline__ float min_f32(float a, float b) {
 template <int C, bool BT>
 struct TabPolicy {
   static constexpr bool kElec = true;   // electrodes in closed form (elec_sums)
+  // coul(N, c) is 0: column N of the c table lies in its zero padding, and column
+  // N of the pair table is an electrode column (or the sentinel), c = 0
+  static constexpr bool kZeroColumnN = true;
   const float2* tab;   // [S*S + 64C]: entry (s, u) at u*S + s; column S is all -inf
   const float* ctab;   // BT: [N*S + 64C] c at u*S + s
   const float* elw;    // [4*ceil(P/4)][64C] electrode weights of the lane sites
@@ -163,6 +166,7 @@ struct TabPolicy {
 template <int C>
 struct PosPolicy {
   static constexpr bool kElec = false;   // electrodes are walked as members
+  static constexpr bool kZeroColumnN = false;
   const float* elw = nullptr;
   float* evc = nullptr;
   const float4* pos;   // [S] per wave: x, y, z, 1 for acceptors / 0 for electrodes
@@ -395,7 +399,7 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
   for (int c = 0; c < C; ++c) walk[c] = elec ? accm[c] : ~0ull;
 
   const int K = p.recompute_interval;
-  float lg_lane = 0.f;    // log2(1 - u_clock) of hop (h & ~63) + lane
+  float lg_lane = 0.f;    // log2(1 - u_clock) * -ln 2 = -ln(1 - u_clock) of hop (h & ~63) + lane
   double ue_lane = 0.0;   // event uniform of that hop
   // next FP64 recompute (a multiple of K; INT32_MAX: none left in this range)
   auto next_multiple = [K](int64_t x) {
@@ -409,7 +413,7 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
     if (h == h_begin || (h & 63) == 0) {
       double uc;
       stream_uniforms(td.seed, td.hop0 + (uint64_t)((h & ~(int64_t)63) + lane), uc, ue_lane);
-      lg_lane = fast_log2((float)(1.0 - uc));
+      lg_lane = fast_log2((float)(1.0 - uc)) * -0.69314718055994531f;
     }
     if (__builtin_expect(h == prehops, 0)) {
       time = 0.0;
@@ -471,7 +475,7 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
     const float lg = readlane_f32(lg_lane, (int)(h & 63));
     const double ue = readlane_f64(ue_lane, (int)(h & 63));
     // exponential waiting time -ln(1-u)/total, FP32 quotient into the FP64 clock
-    const double dt = (double)(lg * -0.69314718055994531f * __builtin_amdgcn_rcpf((float)total));
+    const double dt = (double)(lg * __builtin_amdgcn_rcpf((float)total));
     const double target = ue * total;
 
     int sel = -1;
@@ -549,14 +553,19 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
     float cs[C], cd[C];
 #pragma unroll
     for (int c = 0; c < C; ++c) {
-      cs[c] = (c == 0 || 64 * c < S) ? pol.coul(src_acc ? src : 0, c) : 0.f;
-      cd[c] = (c == 0 || 64 * c < S) ? pol.coul(dst_acc ? dst : 0, c) : 0.f;
+      if constexpr (Pol::kZeroColumnN) {   // column N holds c = 0: no select per lane
+        cs[c] = (c == 0 || 64 * c < S) ? pol.coul(src_acc ? src : N, c) : 0.f;
+        cd[c] = (c == 0 || 64 * c < S) ? pol.coul(dst_acc ? dst : N, c) : 0.f;
+      } else {
+        cs[c] = (c == 0 || 64 * c < S) && src_acc ? pol.coul(src, c) : 0.f;
+        cd[c] = (c == 0 || 64 * c < S) && dst_acc ? pol.coul(dst, c) : 0.f;
+      }
     }
 #pragma unroll
     for (int c = 0; c < C; ++c) {
       if (src_acc && (src >> 6) == c) occ[c] &= ~(1ull << (src & 63));
       if (dst_acc && (dst >> 6) == c) occ[c] |= 1ull << (dst & 63);
-      se[c] = (se[c] - (src_acc ? cs[c] : 0.f)) + (dst_acc ? cd[c] : 0.f);
+      se[c] = (se[c] - cs[c]) + cd[c];
       if (!src_acc && lane + 64 * c == src) eo[c] -= 1;
       if (!dst_acc && lane + 64 * c == dst) eo[c] += 1;
     }
