@@ -12,6 +12,9 @@ holds the restatements:
 * :func:`go_simulate` -- ``simulate`` / ``simulateRecordPlus``
   (goSimulation/simulation.go:194-325, :327-432) in FP32.
 * :func:`philox4x32_10`, :func:`stream_uniforms` -- the engine's RNG contract.
+* :func:`prob_simulate` -- ``probSimulate``
+  (goSimulation/probabilitySimulation.go:53-157), the mean-field integrator.
+* :func:`average_current` -- ``calcAverageCurrent`` (simulation.go:671-736).
 
 ``oracle.pyref`` repeats the numba restatement in pure Python for tiny cases
 (a second, independent reading of the same reference lines).
@@ -73,6 +76,14 @@ def lib():
             _dp, _dp, _dp, _dp, _u8p, C.c_int64, _u64p, C.c_uint64, C.c_int64, C.c_int, C.c_int,
             C.c_int64, C.c_int, _dp, _dp, _u8p, _i32p]
         L.oracle_go_ensemble.restype = C.c_int
+        L.oracle_prob_simulate.argtypes = [
+            C.c_int64, C.c_int64, C.c_double, C.c_double, C.c_double, C.c_double,
+            _dp, _dp, _dp, _dp, _dp, _dp, C.c_int64, C.c_int, _dp, _dp]
+        L.oracle_prob_simulate.restype = C.c_double
+        L.oracle_average_current.argtypes = [
+            C.c_int64, C.c_int64, C.c_double, C.c_double, C.c_double, C.c_double,
+            _dp, _dp, _dp, _dp, _dp, _dp]
+        L.oracle_average_current.restype = None
         L.oracle_max_threads.argtypes = []
         L.oracle_max_threads.restype = C.c_int
         _lib = L
@@ -212,6 +223,46 @@ def go_ensemble(N, P, nu, kT, I_0, R, distances, E_constant, site_energies, tran
                              int(cache_max_bytes), int(nthreads), _p(eo, _dp), _p(tm, _dp),
                              _p(occ, _u8p), _p(st, _i32p))
     return eo, tm, occ, st
+
+
+class ProbResult:
+    pass
+
+
+def prob_simulate(N, P, nu, kT, I_0, R, distances, E_constant, site_energies,
+                  transitions_constant, occupation, hops, record=False) -> ProbResult:
+    """Run the probSimulate restatement (FP64).  ``occupation`` is the fractional
+    start state (wrapperSimulateProbability passes all 0.5)."""
+    S = N + P
+    d = np.ascontiguousarray(distances, dtype=np.float64)
+    tc = np.ascontiguousarray(transitions_constant, dtype=np.float64)
+    E = np.ascontiguousarray(E_constant, dtype=np.float64)
+    se = np.array(site_energies, dtype=np.float64, copy=True)
+    occ = np.array(occupation, dtype=np.float64, copy=True)
+    eo = np.zeros(P, dtype=np.float64)
+    traffic = np.zeros((S, S)) if record else None
+    avg = np.zeros(N) if record else None
+    t = lib().oracle_prob_simulate(N, P, nu, kT, I_0, R, _p(occ, _dp), _p(d, _dp), _p(E, _dp),
+                                   _p(tc, _dp), _p(eo, _dp), _p(se, _dp), int(hops),
+                                   int(bool(record)), _p(traffic, _dp), _p(avg, _dp))
+    r = ProbResult()
+    r.time, r.occupation, r.electrode_occupation, r.site_energies = t, occ, eo, se
+    r.traffic, r.average_occupation = traffic, avg
+    return r
+
+
+def average_current(N, P, nu, kT, I_0, R, distances, E_constant, site_energies,
+                    transitions_constant, occupation) -> np.ndarray:
+    """calcAverageCurrent restatement: currents (P,) at the mean occupation."""
+    d = np.ascontiguousarray(distances, dtype=np.float64)
+    tc = np.ascontiguousarray(transitions_constant, dtype=np.float64)
+    E = np.ascontiguousarray(E_constant, dtype=np.float64)
+    se = np.ascontiguousarray(site_energies, dtype=np.float64)
+    occ = np.ascontiguousarray(occupation, dtype=np.float64)
+    out = np.zeros(P, dtype=np.float64)
+    lib().oracle_average_current(N, P, nu, kT, I_0, R, _p(occ, _dp), _p(d, _dp), _p(E, _dp),
+                                 _p(tc, _dp), _p(se, _dp), _p(out, _dp))
+    return out
 
 
 def max_threads() -> int:

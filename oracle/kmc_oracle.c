@@ -26,6 +26,17 @@
  *                      Float32 are replaced by the same uniform stream.
  *   oracle_*_ensemble  OpenMP ensemble drivers (one trajectory per thread)
  *                      used as the CPU baseline.
+ *   oracle_prob_simulate
+ *                      Line-by-line restatement of probSimulate,
+ *                      goSimulation/probabilitySimulation.go:53-157 with
+ *                      calcProbTransitions :8-38 and probTransitionPossible
+ *                      :40-50: the deterministic mean-field integrator behind
+ *                      wrapperSimulateProbability (simulationWrapper.go:235-250),
+ *                      FP64.
+ *   oracle_average_current
+ *                      Restatement of calcAverageCurrent, simulation.go:671-736,
+ *                      with its FP32/FP64 mix (inputs cast to float32 as the Go
+ *                      wrapper does, simulationWrapper.go:116-122).
  *
  * Build: oracle/Makefile, gcc -O3 -march=x86-64-v3 -ffp-contract=off.  Contraction
  * must stay off: numba (LLVM without fast-math) evaluates every a*b+c as a
@@ -535,6 +546,167 @@ int oracle_go_ensemble(int64_t N, int64_t P, double nu, double kT, double I_0, d
         free(se);
     }
     return bad ? ERR_ZERO_TOTAL : OK;
+}
+
+/* ------------------------------------------------------------------ */
+/* Mean-field estimators (probabilitySimulation.go, simulation.go)      */
+/* ------------------------------------------------------------------ */
+/* probTransitionPossible, probabilitySimulation.go:40-50 */
+static inline double prob_possible(int64_t i, int64_t j, int64_t NSites, const double* occ) {
+    if (i >= NSites && j >= NSites) return 0;
+    if (i >= NSites) return 1 - occ[j];
+    if (j >= NSites) return occ[i];
+    return (1 - occ[j]) * occ[i];
+}
+
+/* probSimulate, probabilitySimulation.go:53-157.  Arrays are S x S row-major
+ * (S = NSites + NElectrodes, the reference's N).  occupation and site_energies
+ * are updated in place; electrode_occupation is zeroed, then accumulated;
+ * traffic and average_occupation accumulate (record only).  Returns the time. */
+double oracle_prob_simulate(int64_t NSites, int64_t NElectrodes, double nu, double kT, double I_0,
+                            double R, double* occupation, const double* distances,
+                            const double* E_constant, const double* transitions_constant,
+                            double* electrode_occupation, double* site_energies, int64_t hops,
+                            int record, double* traffic, double* average_occupation) {
+    const int64_t N = NSites + NElectrodes;
+    double* transitions = (double*)calloc((size_t)(N * N + 1), sizeof(double));
+    double* difference = (double*)calloc((size_t)(NSites + 1), sizeof(double));
+    if (!transitions || !difference) {
+        free(transitions);
+        free(difference);
+        return NAN;
+    }
+    for (int64_t i = 0; i < NElectrodes; i++) electrode_occupation[i] = 0.0;
+    double time = 0;
+    for (int64_t hop = 0; hop < hops; hop++) {
+        for (int64_t i = 0; i < NSites; i++) {
+            difference[i] = 0;
+            double acceptor_interaction = 0;
+            for (int64_t j = 0; j < NSites; j++)
+                if (j != i) acceptor_interaction += (1 - occupation[j]) / distances[i * N + j];
+            site_energies[i] = E_constant[i] - I_0 * R * acceptor_interaction;
+        }
+        /* calcProbTransitions, :8-38 */
+        double tot_rates = 0.0;
+        for (int64_t i = 0; i < N; i++)
+            for (int64_t j = 0; j < N; j++) {
+                const double base_prob = prob_possible(i, j, NSites, occupation);
+                double dE;
+                if (i < NSites && j < NSites)
+                    dE = site_energies[j] - site_energies[i] - I_0 * R / distances[i * N + j];
+                else
+                    dE = site_energies[j] - site_energies[i];
+                double t;
+                if (dE > 0)
+                    t = base_prob * nu * exp(-dE / kT);
+                else
+                    t = base_prob * nu;
+                t *= transitions_constant[i * N + j];
+                transitions[i * N + j] = t;
+                tot_rates += t;
+                if (i < NSites) difference[i] -= t;
+                if (j < NSites) difference[j] += t;
+            }
+        double max_change = 1.0;
+        for (int64_t i = 0; i < NSites; i++) {
+            const double newVal = occupation[i] + difference[i] / tot_rates;
+            if (newVal < 0) {
+                const double req_change = occupation[i] / -(difference[i] / tot_rates);
+                if (req_change < max_change) max_change = req_change;
+            }
+            if (newVal > 1) {
+                const double req_change = (1 - occupation[i]) / (difference[i] / tot_rates);
+                if (req_change < max_change) max_change = req_change;
+            }
+        }
+        const double time_step = 0.98 * (max_change) / tot_rates;
+        time += time_step;
+        if (record)
+            for (int64_t i = 0; i < NSites; i++) average_occupation[i] += occupation[i] * time_step;
+        for (int64_t i = 0; i < N; i++)
+            for (int64_t j = 0; j < N; j++) {
+                if (i >= NSites && j >= NSites) break;
+                const double rate = transitions[i * N + j] * max_change / tot_rates;
+                if (i < NSites)
+                    occupation[i] -= rate;
+                else
+                    electrode_occupation[i - NSites] -= rate;
+                if (j < NSites)
+                    occupation[j] += rate;
+                else
+                    electrode_occupation[j - NSites] += rate;
+                if (record) {
+                    traffic[i * N + j] += rate;
+                    traffic[j * N + i] -= rate;
+                }
+            }
+        for (int64_t i = 0; i < NSites; i++) {
+            if (occupation[i] < 0) occupation[i] = 0;
+            if (occupation[i] > 1) occupation[i] = 1;
+        }
+    }
+    free(transitions);
+    free(difference);
+    return time;
+}
+
+/* calcAverageCurrent, simulation.go:671-736, after the wrapper's float32 casts of
+ * nu, kT, I_0, R, distances, E_constant, transitions_constant and site_energies
+ * (only the electrode entries [NSites:] are read).  occupation is the
+ * time-averaged occupation (float64); currents[e] = eoDifference / time_step. */
+void oracle_average_current(int64_t NSites, int64_t NElectrodes, double nu64, double kT64,
+                            double I064, double R64, const double* occupation,
+                            const double* distances64, const double* E_constant64,
+                            const double* tc64, const double* site_energies64,
+                            double* currents) {
+    const int64_t N = NSites + NElectrodes;
+    const float nu = (float)nu64, kT = (float)kT64, I_0 = (float)I064, R = (float)R64;
+    float* se = (float*)calloc((size_t)(N + 1), sizeof(float));
+    double* transitions = (double*)calloc((size_t)(N * N + 1), sizeof(double));
+    double* eoDifference = (double*)calloc((size_t)(NElectrodes + 1), sizeof(double));
+    if (!se || !transitions || !eoDifference) {
+        free(se);
+        free(transitions);
+        free(eoDifference);
+        return;
+    }
+    for (int64_t i = 0; i < NSites; i++) {
+        double acceptor_interaction = 0;
+        for (int64_t j = 0; j < NSites; j++)
+            if (j != i) acceptor_interaction += (1 - occupation[j]) / (double)(float)distances64[i * N + j];
+        se[i] = (float)E_constant64[i] - I_0 * R * (float)acceptor_interaction;
+    }
+    for (int64_t e = 0; e < NElectrodes; e++) se[NSites + e] = (float)site_energies64[NSites + e];
+    double tot_rates = 0.0;
+    for (int64_t i = 0; i < N; i++)
+        for (int64_t j = 0; j < N; j++) {
+            const double base_prob = prob_possible(i, j, NSites, occupation);
+            float dE;
+            if (i < NSites && j < NSites)
+                dE = se[j] - se[i] - I_0 * R / (float)distances64[i * N + j];
+            else
+                dE = se[j] - se[i];
+            double t;
+            if (dE > 0)
+                t = base_prob * (double)nu * exp((double)(-dE / kT));
+            else
+                t = base_prob * (double)nu;
+            t *= (double)(float)tc64[i * N + j];
+            transitions[i * N + j] = t;
+            tot_rates += t;
+        }
+    const double time_step = 0.98 / tot_rates;
+    for (int64_t i = 0; i < N; i++)
+        for (int64_t j = 0; j < N; j++) {
+            if (i >= NSites && j >= NSites) break;
+            const double rate = transitions[i * N + j] / tot_rates;
+            if (i >= NSites) eoDifference[i - NSites] -= rate;
+            if (j >= NSites) eoDifference[j - NSites] += rate;
+        }
+    for (int64_t e = 0; e < NElectrodes; e++) currents[e] = eoDifference[e] / time_step;
+    free(se);
+    free(transitions);
+    free(eoDifference);
 }
 
 int oracle_max_threads(void) {
