@@ -19,6 +19,10 @@
  *                           ctypes callers (goSimulation/pythonBind.py:49-90,
  *                           goSimulation/parrallelSimulationBind.py:50-79) can load
  *                           this library unchanged.
+ *   kmc_meanfield_simulate  probSimulate, goSimulation/probabilitySimulation.go:53-157
+ *                           (behind wrapperSimulateProbability, simulationWrapper.go:235-250)
+ *   kmc_average_current     calcAverageCurrent, goSimulation/simulation.go:671-736
+ *                           (the estimator of wrapperSimulateCombined, :112-125)
  */
 #ifndef KMC_DN_AMD_H
 #define KMC_DN_AMD_H
@@ -80,6 +84,35 @@ int kmc_simulate_discrete(int64_t N, int64_t P, double nu, double kT, double I_0
                           int record, double* traffic, double* occupations_in_time,
                           const double* uniforms, uint64_t seed, uint64_t hop0,
                           int32_t* events_out, double* totals_out, double* time_out);
+
+/*
+ * probSimulate (goSimulation/probabilitySimulation.go:53-157): the Go library's
+ * deterministic mean-field integrator, `steps` steps of n networks at once
+ * (network i: layout layouts[i], energy set esets[i] of that layout), FP64.
+ * occupation: n*KMC_NMAX fractional occupations, updated in place (the reference
+ * starts wrapperSimulateProbability from 0.5 everywhere); electrode_occupation:
+ * n*KMC_PMAX fractional hop counts, zeroed then accumulated; site_energies
+ * (n*KMC_NMAX, optional): acceptor energies of the last step; time (n,
+ * optional); average_occupation (n*KMC_NMAX, optional): occupation x time;
+ * traffic (optional): n blocks of Smax*Smax doubles (Smax = the largest S of the
+ * n networks), each holding its network's S x S net traffic row-major from the
+ * block start.  Sums run in another order than the reference's loops: results
+ * agree to rounding.
+ */
+int kmc_meanfield_simulate(int32_t n, const int32_t* layouts, const int32_t* esets, int64_t steps,
+                           double* occupation, double* electrode_occupation, double* site_energies,
+                           double* time, double* average_occupation, double* traffic);
+
+/*
+ * calcAverageCurrent (goSimulation/simulation.go:671-736) of n networks at the
+ * time-averaged occupations occupation (n*KMC_NMAX): currents (n*KMC_PMAX) =
+ * each electrode's net inflow rate over the total rate, divided by the
+ * reference's time step 0.98 / total.  Site energies and energy differences in
+ * FP32, rates in FP64, as the reference computes them after its wrapper's
+ * float32 casts.
+ */
+int kmc_average_current(int32_t n, const int32_t* layouts, const int32_t* esets,
+                        const double* occupation, double* currents);
 
 /* ---- device-resident batches (parallelSimulations, simulationWrapper.go:274-315) ---- */
 

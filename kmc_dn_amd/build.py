@@ -32,6 +32,7 @@ SOURCES = {
     "kmc_plain.hip": ["-ffp-contract=off"],
     "kmc_batch.hip": [],
     "kmc_team.hip": [],
+    "kmc_meanfield.hip": ["-ffp-contract=off"],
     "kmc_capi.hip": [],
     "kmc_probe.hip": [],
     "kmc_goshim.cpp": [],

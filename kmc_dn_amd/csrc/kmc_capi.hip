@@ -35,6 +35,10 @@ size_t pos_lds_bytes(int C, int W);
 template <int C, int T, bool REC, int FORM>
 __global__ void kmc_team_kernel(BatchParams);
 size_t team_lds_bytes(int form, int C, int T, int S, int N, int P);
+template <int C>
+__global__ void kmc_meanfield_kernel(MeanfieldParams);
+template <int C>
+__global__ void kmc_avg_current_kernel(MeanfieldParams);
 
 namespace {
 
@@ -97,7 +101,7 @@ struct Layout {
   ~Layout() {
     for (DevBuf* b : eset_bufs) delete b;
   }
-  DevBuf tab32, tabB, ccol, elw, coul64, tc64, ird64, invd64, pos4;
+  DevBuf tab32, tabB, ccol, elw, coul64, tc64, ird64, invd64, dist64, pos4;
   LayoutDev dev{};
 };
 
@@ -276,6 +280,7 @@ int layout_create_locked(int64_t N, int64_t P, double nu, double kT, double I_0,
   if ((st = upload(L->tab32, tab)) || (st = upload(L->tabB, tabB)) || (st = upload(L->ccol, ccol)) ||
       (st = upload(L->elw, elw)) ||
       (st = upload(L->coul64, coul)) || (st = upload(L->tc64, tc64)) ||
+      (st = upload(L->dist64, std::vector<double>(d, d + (size_t)S * S))) ||
       (st = upload(L->ird64, ird)) || (st = upload(L->invd64, invd))) {
     delete L;
     return st;
@@ -298,6 +303,7 @@ int layout_create_locked(int64_t N, int64_t P, double nu, double kT, double I_0,
   D.elw = L->elw.as<float>();
   D.coul64 = L->coul64.as<double>();
   D.tc64 = L->tc64.as<double>();
+  D.dist64 = L->dist64.as<double>();
   D.ird64 = L->ird64.as<double>();
   D.invd64 = L->invd64.as<double>();
   Engine& E = eng();
@@ -1295,6 +1301,105 @@ int kmc_batch_destroy(int32_t batch) {
   delete B;
   E.batches[batch] = nullptr;
   return KMC_OK;
+}
+
+}  // extern "C" (reopened below)
+
+namespace {
+
+// One wave per network; n networks of layouts[i] / esets[i].  avg_current: the
+// calcAverageCurrent evaluation (currents into eo_out), else probSimulate for
+// `steps` steps (occupation in/out).
+int meanfield_locked(bool avg_current, int32_t n, const int32_t* layouts, const int32_t* esets,
+                     int64_t steps, double* occupation, double* eo_out, double* se_out,
+                     double* time_out, double* avg_out, double* traffic_out) {
+  Engine& E = eng();
+  int st = ensure_init();
+  if (st) return st;
+  if (n < 0 || (n > 0 && (!layouts || !esets || !occupation || !eo_out)))
+    return fail(KMC_ERR_ARGS, "null argument");
+  if (steps < 0) return fail(KMC_ERR_ARGS, "steps must be >= 0");
+  if (n == 0) return KMC_OK;
+  int smax = 0;
+  for (int32_t i = 0; i < n; ++i) {
+    const int32_t l = layouts[i], e = esets[i];
+    if (l < 0 || l >= (int)E.layouts.size() || !E.layouts[l])
+      return fail(KMC_ERR_ARGS, "network %d: no layout %d", i, l);
+    if (e < 0 || e >= (int)E.eset_layout.size() || E.eset_layout[e] != l)
+      return fail(KMC_ERR_ARGS, "network %d: energy set %d does not belong to layout %d", i, e, l);
+    smax = std::max(smax, E.layouts[l]->S);
+  }
+  if ((st = sync_registry())) return st;
+  const int C = (smax + 63) / 64;
+  const size_t na = (size_t)n * NMAX, ne = (size_t)n * PMAX;
+  const int64_t tstride = (int64_t)smax * smax;
+  DevBuf ids, occ, eo, se, tm, avg, tr;
+  std::vector<int32_t> both((size_t)2 * n);
+  std::copy(layouts, layouts + n, both.begin());
+  std::copy(esets, esets + n, both.begin() + n);
+  if ((st = upload(ids, both)) || (st = occ.alloc(na * sizeof(double))) ||
+      (st = eo.alloc(ne * sizeof(double))) || (st = tm.alloc((size_t)n * sizeof(double))) ||
+      (se_out && (st = se.alloc(na * sizeof(double)))) ||
+      (avg_out && (st = avg.alloc(na * sizeof(double)))) ||
+      (traffic_out && (st = tr.alloc((size_t)n * tstride * sizeof(double)))))
+    return st;
+  HIPOK(hipMemcpyAsync(occ.p, occupation, na * sizeof(double), hipMemcpyHostToDevice, E.stream));
+  HIPOK(hipMemsetAsync(eo.p, 0, ne * sizeof(double), E.stream));
+  if (traffic_out) HIPOK(hipMemsetAsync(tr.p, 0, (size_t)n * tstride * sizeof(double), E.stream));
+  MeanfieldParams mp{};
+  mp.layouts = E.layouts_dev.as<LayoutDev>();
+  mp.esets = E.esets_dev.as<EsetDev>();
+  mp.layout_of = ids.as<int32_t>();
+  mp.eset_of = ids.as<int32_t>() + n;
+  mp.occ = occ.as<double>();
+  mp.eo = eo.as<double>();
+  mp.se = se_out ? se.as<double>() : nullptr;
+  mp.time = tm.as<double>();
+  mp.avg = avg_out ? avg.as<double>() : nullptr;
+  mp.traffic = traffic_out ? tr.as<double>() : nullptr;
+  mp.tstride = tstride;
+  mp.steps = steps;
+  mp.n = n;
+  using K = void (*)(MeanfieldParams);
+  K k = nullptr;
+  if (avg_current)
+    k = C == 1 ? (K)kmc_avg_current_kernel<1> : C == 2 ? (K)kmc_avg_current_kernel<2> : (K)kmc_avg_current_kernel<3>;
+  else
+    k = C == 1 ? (K)kmc_meanfield_kernel<1> : C == 2 ? (K)kmc_meanfield_kernel<2> : (K)kmc_meanfield_kernel<3>;
+  hipLaunchKernelGGL(k, dim3(n), dim3(64), 0, E.stream, mp);
+  HIPOK(hipGetLastError());
+  if (!avg_current) {
+    HIPOK(hipMemcpyAsync(occupation, occ.p, na * sizeof(double), hipMemcpyDeviceToHost, E.stream));
+    if (time_out) HIPOK(hipMemcpyAsync(time_out, tm.p, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, E.stream));
+    if (se_out) HIPOK(hipMemcpyAsync(se_out, se.p, na * sizeof(double), hipMemcpyDeviceToHost, E.stream));
+    if (avg_out) HIPOK(hipMemcpyAsync(avg_out, avg.p, na * sizeof(double), hipMemcpyDeviceToHost, E.stream));
+    if (traffic_out)
+      HIPOK(hipMemcpyAsync(traffic_out, tr.p, (size_t)n * tstride * sizeof(double), hipMemcpyDeviceToHost, E.stream));
+  }
+  HIPOK(hipMemcpyAsync(eo_out, eo.p, ne * sizeof(double), hipMemcpyDeviceToHost, E.stream));
+  HIPOK(hipStreamSynchronize(E.stream));
+  return KMC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int kmc_meanfield_simulate(int32_t n, const int32_t* layouts, const int32_t* esets, int64_t steps,
+                           double* occupation, double* electrode_occupation, double* site_energies,
+                           double* time, double* average_occupation, double* traffic) {
+  Engine& E = eng();
+  std::lock_guard<std::mutex> lk(E.mu);
+  return meanfield_locked(false, n, layouts, esets, steps, occupation, electrode_occupation,
+                          site_energies, time, average_occupation, traffic);
+}
+
+int kmc_average_current(int32_t n, const int32_t* layouts, const int32_t* esets,
+                        const double* occupation, double* currents) {
+  Engine& E = eng();
+  std::lock_guard<std::mutex> lk(E.mu);
+  return meanfield_locked(true, n, layouts, esets, 0, const_cast<double*>(occupation), currents,
+                          nullptr, nullptr, nullptr, nullptr);
 }
 
 int kmc_simulate_discrete(int64_t N, int64_t P, double nu, double kT, double I_0, double R,

@@ -15,6 +15,10 @@
 //   * the occupation slice is not written (the Go code never writes it back).
 // Start states follow the Go wrappers: empty for wrapperSimulate* (:134-141), the
 // given occupation (> 0 means occupied) for parallelSimulations (:256-263).
+// wrapperSimulateProbability runs the mean-field integrator (kmc_meanfield_simulate)
+// and wrapperSimulateCombined the hop loop plus calcAverageCurrent
+// (kmc_average_current); analyzeStateOverlap, a diagnostic of the Go cache,
+// returns NaN.
 // Errors cannot be returned through these signatures: the functions return NaN
 // (time) or -1 (parallelSimulations) and kmc_last_error() says why; the Go library
 // would panic instead.
@@ -47,7 +51,8 @@ double run_one(int64_t NSites, int64_t NElectrodes, double prune, double nu, dou
                double R, const GoSlice& occupation, const GoSlice& distances,
                const GoSlice& E_constant, const GoSlice& transitions_constant,
                GoSlice electrode_occupation, const GoSlice& site_energies, int64_t hops_raw,
-               bool record, GoSlice traffic, GoSlice average_occupation, bool from_empty) {
+               bool record, GoSlice traffic, GoSlice average_occupation, bool from_empty,
+               bool combined = false) {
   // The reference's ctypes binding passes hops as a 32-bit c_int to this 64-bit
   // argument; only the low 32 bits are defined, so read them as a signed int32.
   const int64_t hops = (int32_t)(uint32_t)hops_raw;
@@ -89,6 +94,15 @@ double run_one(int64_t NSites, int64_t NElectrodes, double prune, double nu, dou
       break;
     if (status != KMC_OK) break;
     for (int64_t e = 0; e < P; ++e) electrode_occupation.data[e] = (double)eo[e];
+    if (combined) {
+      // simulateCombined (simulation.go:592-669): the hop loop's time-averaged
+      // occupation through calcAverageCurrent (:671-736), times the elapsed time
+      if (!(time > 0)) break;
+      std::vector<double> avg(KMC_NMAX, 0.0), cur(KMC_PMAX, 0.0);
+      for (int64_t i = 0; i < N; ++i) avg[i] = ot[i] / time;
+      if (kmc_average_current(1, &lay, &es, avg.data(), cur.data()) != KMC_OK) break;
+      for (int64_t e = 0; e < P; ++e) electrode_occupation.data[e] = cur[e] * time;
+    }
     if (record) {
       int64_t stride = 0;
       if (kmc_batch_traffic(b, nullptr, &stride) != KMC_OK) break;
@@ -156,17 +170,74 @@ double wrapperSimulateRecordPlus(int64_t NSites, int64_t NElectrodes, double nu,
                  average_occupation, true);
 }
 
-// simulationWrapper.go:112-125, :171-216, :218-233: not hop loops of this engine.
-double wrapperSimulateCombined(int64_t, int64_t, double, double, double, double, GoSlice, GoSlice,
-                               GoSlice, GoSlice, GoSlice, GoSlice, int64_t, uint8_t, GoSlice,
-                               GoSlice) {
-  return NAN;
+// simulationWrapper.go:112-125: the FP32 hop loop from an empty network, then
+// calcAverageCurrent at its time-averaged occupation; electrode_occupation =
+// those currents x time.  simulateCombined records traffic and occupation time
+// whatever `record` says (simulation.go:646-654).
+double wrapperSimulateCombined(int64_t NSites, int64_t NElectrodes, double nu, double kT,
+                               double I_0, double R, GoSlice occupation, GoSlice distances,
+                               GoSlice E_constant, GoSlice transitions_constant,
+                               GoSlice electrode_occupation, GoSlice site_energies, int64_t hops,
+                               uint8_t record, GoSlice traffic, GoSlice average_occupation) {
+  (void)record;
+  return run_one(NSites, NElectrodes, 0.0, nu, kT, I_0, R, occupation, distances, E_constant,
+                 transitions_constant, electrode_occupation, site_energies, hops, true, traffic,
+                 average_occupation, true, true);
 }
-double wrapperSimulateProbability(int64_t, int64_t, double, double, double, double, GoSlice,
-                                  GoSlice, GoSlice, GoSlice, GoSlice, GoSlice, int64_t, uint8_t,
-                                  GoSlice, GoSlice) {
-  return NAN;
+
+// simulationWrapper.go:235-250: every acceptor at occupation 0.5 (written into the
+// caller's slice, as the Go code does), then probSimulate
+// (probabilitySimulation.go:53-157) for `hops` steps; occupation, site_energies
+// [0:N] and electrode_occupation are written back, traffic and
+// average_occupation accumulate when `record`.
+double wrapperSimulateProbability(int64_t NSites, int64_t NElectrodes, double nu, double kT,
+                                  double I_0, double R, GoSlice occupation, GoSlice distances,
+                                  GoSlice E_constant, GoSlice transitions_constant,
+                                  GoSlice electrode_occupation, GoSlice site_energies,
+                                  int64_t hops_raw, uint8_t record, GoSlice traffic,
+                                  GoSlice average_occupation) {
+  const int64_t hops = (int32_t)(uint32_t)hops_raw;
+  const int64_t N = NSites, P = NElectrodes, S = N + P;
+  if (kmc_init(-1) != KMC_OK) return NAN;
+  if (N < 0 || P < 0 || hops < 0 || !slice_ok(distances, S * S) ||
+      !slice_ok(transitions_constant, S * S) || !slice_ok(E_constant, N) ||
+      !slice_ok(site_energies, S) || (P > 0 && !slice_ok(electrode_occupation, P)) ||
+      (N > 0 && !slice_ok(occupation, N)))
+    return NAN;
+  int32_t lay = -1, es = -1;
+  if (kmc_layout_create(N, P, nu, kT, I_0, R, distances.data, transitions_constant.data, 0.0,
+                        &lay) != KMC_OK)
+    return NAN;
+  double result = NAN;
+  do {
+    if (kmc_eset_create(lay, 1, E_constant.data, site_energies.data + N, &es) != KMC_OK) break;
+    for (int64_t j = 0; j < N; ++j) occupation.data[j] = 0.5;
+    std::vector<double> occ(KMC_NMAX, 0.0), eo(KMC_PMAX, 0.0), se(KMC_NMAX, 0.0);
+    std::vector<double> avg(record ? KMC_NMAX : 0), tr(record ? (size_t)(S * S) : 0);
+    for (int64_t j = 0; j < N; ++j) occ[j] = occupation.data[j];
+    double time = 0.0;
+    if (kmc_meanfield_simulate(1, &lay, &es, hops, occ.data(), eo.data(), se.data(), &time,
+                               record ? avg.data() : nullptr, record ? tr.data() : nullptr) !=
+        KMC_OK)
+      break;
+    for (int64_t j = 0; j < N; ++j) occupation.data[j] = occ[j];
+    if (hops > 0)
+      for (int64_t j = 0; j < N; ++j) site_energies.data[j] = se[j];
+    for (int64_t e = 0; e < P; ++e) electrode_occupation.data[e] = eo[e];
+    if (record) {
+      if (slice_ok(traffic, S * S))
+        for (int64_t q = 0; q < S * S; ++q) traffic.data[q] += tr[q];
+      if (slice_ok(average_occupation, N))
+        for (int64_t i = 0; i < N; ++i) average_occupation.data[i] += avg[i];
+    }
+    result = time;
+  } while (false);
+  kmc_layout_destroy(lay);
+  return result;
 }
+
+// simulationWrapper.go:171-216 prints state-overlap statistics of the Go cache
+// and returns 0: a diagnostic of that cache, which this engine does not have.
 double analyzeStateOverlap(int64_t, int64_t, double, double, double, double, GoSlice, GoSlice,
                            GoSlice, GoSlice, GoSlice, GoSlice, int64_t, uint8_t, GoSlice, GoSlice) {
   return NAN;

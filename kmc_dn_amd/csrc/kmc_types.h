@@ -50,11 +50,34 @@ struct LayoutDev {
   const double* tc64;     // [S*S] transitions_constant
   const double* ird64;    // [S*S] (I_0*R)/d[i][j] (acceptor pairs; 0 elsewhere)
   const double* invd64;   // [N*N] 1.0/d[i][j] (0 on the diagonal)
+  const double* dist64;   // [S*S] distances as given (mean-field estimators)
 };
+
+
 
 struct EsetDev {
   const double* E;   // [N] E_constant
   const double* V;   // [P] electrode voltages (site_energies[N:])
+};
+
+// Mean-field estimators of the Go library: probSimulate
+// (goSimulation/probabilitySimulation.go:53-157) and calcAverageCurrent
+// (goSimulation/simulation.go:671-736), one wave per network.
+struct MeanfieldParams {
+  const LayoutDev* layouts;
+  const EsetDev* esets;
+  const int32_t* layout_of;   // [n]
+  const int32_t* eset_of;     // [n]
+  double* occ;                // [n][NMAX] fractional occupation (in; probSimulate: out)
+  double* eo;                 // [n][PMAX] electrode occupation (probSimulate) or currents
+  double* se;                 // [n][NMAX] acceptor energies of the last step, or null
+  double* time;               // [n] or null
+  double* avg;                // [n][NMAX] occupation x time, or null
+  double* traffic;            // [n][tstride] net traffic (S x S row-major), or null
+  int64_t tstride;
+  int64_t steps;
+  int32_t n;
+  int32_t pad;
 };
 
 struct TrajDev {

@@ -248,6 +248,59 @@ class Batch:
             pass
 
 
+class MeanField:
+    """Result of meanfield_simulate for n networks."""
+    occupation: np.ndarray            # (n, NMAX) fractional occupations
+    electrode_occupation: np.ndarray  # (n, PMAX) fractional hop counts
+    site_energies: np.ndarray         # (n, NMAX) acceptor energies of the last step
+    time: np.ndarray                  # (n,)
+    average_occupation: np.ndarray    # (n, NMAX) occupation x time, or None
+    traffic: np.ndarray               # (n, Smax, Smax) net traffic, or None
+
+
+def meanfield_simulate(layouts, esets, steps, occupation, record=False) -> MeanField:
+    """probSimulate (goSimulation/probabilitySimulation.go:53-157) on the GPU for
+    n networks: Layout objects layouts[i] with energy sets esets[i], fractional
+    start occupations (n, N_i or NMAX).  Sums run in another order than the
+    reference's loops (rounding-level differences)."""
+    layouts = list(layouts)
+    lay = np.ascontiguousarray([x.id for x in layouts], dtype=np.int32)
+    es = np.ascontiguousarray(np.atleast_1d(esets), dtype=np.int32)
+    n = lay.size
+    occ0 = np.atleast_2d(np.asarray(occupation, dtype=np.float64))
+    occ = np.zeros((n, L.NMAX))
+    occ[:, :occ0.shape[1]] = occ0
+    eo = np.zeros((n, L.PMAX))
+    se = np.zeros((n, L.NMAX))
+    t = np.zeros(n)
+    avg = np.zeros((n, L.NMAX)) if record else None
+    smax = max(x.S for x in layouts)
+    tr = np.zeros((n, smax, smax)) if record else None
+    L.check(L.load().kmc_meanfield_simulate(
+        n, L.ptr(lay, L._i32p), L.ptr(es, L._i32p), int(steps), L.ptr(occ, L._dp),
+        L.ptr(eo, L._dp), L.ptr(se, L._dp), L.ptr(t, L._dp), L.ptr(avg, L._dp), L.ptr(tr, L._dp)))
+    r = MeanField()
+    r.occupation, r.electrode_occupation, r.site_energies, r.time = occ, eo, se, t
+    r.average_occupation, r.traffic = avg, tr
+    return r
+
+
+def average_current(layouts, esets, occupation) -> np.ndarray:
+    """calcAverageCurrent (goSimulation/simulation.go:671-736) on the GPU at the
+    time-averaged occupations (n, N_i or NMAX) of Layout objects layouts[i] with
+    energy sets esets[i]; returns currents (n, PMAX)."""
+    lay = np.ascontiguousarray([x.id for x in layouts], dtype=np.int32)
+    es = np.ascontiguousarray(np.atleast_1d(esets), dtype=np.int32)
+    n = lay.size
+    occ0 = np.atleast_2d(np.asarray(occupation, dtype=np.float64))
+    occ = np.zeros((n, L.NMAX))
+    occ[:, :occ0.shape[1]] = occ0
+    cur = np.zeros((n, L.PMAX))
+    L.check(L.load().kmc_average_current(n, L.ptr(lay, L._i32p), L.ptr(es, L._i32p),
+                                         L.ptr(occ, L._dp), L.ptr(cur, L._dp)))
+    return cur
+
+
 def simulate_discrete(N, P, nu, kT, I_0, R, distances, E_constant, site_energies,
                       transitions_constant, occupation, electrode_occupation, hops, record=False,
                       uniforms=None, seed=0, hop0=0, log=False):

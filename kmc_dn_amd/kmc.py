@@ -261,13 +261,75 @@ class kmc_dn:
 
     def go_simulation(self, hops=1E5, prehops=0, goSpecificFunction="wrapperSimulateRecord",
                       record=False, prune_threshold=0):
-        """FP32 batched GPU kernel with the Go library's semantics (:473-511)."""
+        """FP32 batched GPU kernel with the Go library's semantics (:473-511).
+
+        goSpecificFunction "wrapperSimulateProbability" runs the mean-field
+        integrator (probSimulate, goSimulation/probabilitySimulation.go:53-157)
+        for `hops` steps, and "wrapperSimulateCombined" the hop loop followed by
+        calcAverageCurrent (simulation.go:592-736).  Their electrode occupations
+        and occupations are fractional; the reference's binding truncates both
+        with int() (goSimulation/pythonBind.py:82-83) before current =
+        electrode_occupation / time, and so does this method.  The untruncated
+        values stay in self.estimator (time, occupation, electrode_occupation)."""
+        if goSpecificFunction in ("wrapperSimulateProbability", "wrapperSimulateCombined"):
+            self._go_estimator(goSpecificFunction, hops, prehops, record)
+            return
         if goSpecificFunction not in ("wrapperSimulate", "wrapperSimulateRecord",
                                       "wrapperSimulateRecordPlus", "wrapperSimulatePruned"):
             raise _lib.KmcError(_lib.ERR_UNSUPPORTED, f"{goSpecificFunction} is not a hop loop")
         self.makeSimulation(prehops=prehops, hops=hops, record=record, back_end="fp32",
                             prune_threshold=prune_threshold if goSpecificFunction ==
                             "wrapperSimulatePruned" else 0.0)
+
+    def _go_estimator(self, name, hops, prehops, record):
+        """makeSimulation (:544-618) with callGoSimulation (pythonBind.py:49-90)
+        bound to one of the Go library's mean-field estimators."""
+        self.reset()
+        if prehops != 0:   # the reference runs them, then the wrapper ignores the state
+            self._fp64_hops(int(prehops), False)
+            self.reset()
+        hops = int(hops)
+        S = self.N + self.P
+        lay = self._layout()
+        es = lay.energy_sets(self.E_constant, self.site_energies[self.N:])
+        try:
+            if name == "wrapperSimulateProbability":
+                r = engine.meanfield_simulate([lay], es[:1], hops, np.full(self.N, 0.5),
+                                              record=record)
+                t = float(r.time[0])
+                occ = r.occupation[0, :self.N]
+                eo = r.electrode_occupation[0, :self.P]
+                traffic = r.traffic[0, :S, :S] if record else None
+                oit = r.average_occupation[0, :self.N] if record else None
+            else:
+                b = engine.Batch(engine.make_trajs(1, lay.id, es[0], self._draw_seed(), hops,
+                                                   np.zeros(self.N, bool)))
+                try:
+                    b.run(precision=_lib.FP32_BATCHED, record=True, traffic=True,
+                          post_hop_record=False)
+                    b.sync()
+                    res = b.results()
+                    _lib.check(int(res.status[0]))
+                    traffic = b.traffic(S)[0]
+                finally:
+                    b.close()
+                t = float(res.time[0])
+                oit = res.occ_time[0, :self.N]
+                occ = engine.bits_to_occupation(res.occ_bits[0], self.N).astype(np.float64)
+                cur = engine.average_current([lay], es[:1], oit / t)
+                eo = cur[0, :self.P] * t
+        finally:
+            engine.Layout.drop_energy_sets(es)
+        self.estimator = type("Estimate", (), {})()
+        self.estimator.time, self.estimator.occupation = t, occ
+        self.estimator.electrode_occupation = eo
+        self.time = t
+        self.occupation = np.array([int(x) for x in occ])
+        self.electrode_occupation = np.array([int(x) for x in eo])
+        if record:
+            self.traffic = traffic
+            self.average_occupation = [x / self.time for x in oit]
+        self.current = self.electrode_occupation / self.time
 
     def makeSimulation(self, simulateFunction=None, preHopFunction=None, prehops=0, hops=1E5,
                        record=False, goSpecificFunction=None, prune_threshold=0.0,
