@@ -554,8 +554,8 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
 #pragma unroll
     for (int c = 0; c < C; ++c) {
       if constexpr (Pol::kZeroColumnN) {   // column N holds c = 0: no select per lane
-        cs[c] = (c == 0 || 64 * c < S) ? pol.coul(src_acc ? src : N, c) : 0.f;
-        cd[c] = (c == 0 || 64 * c < S) ? pol.coul(dst_acc ? dst : N, c) : 0.f;
+        cs[c] = (c == 0 || 64 * c < S) ? pol.coul(min((unsigned)src, (unsigned)N), c) : 0.f;
+        cd[c] = (c == 0 || 64 * c < S) ? pol.coul(min((unsigned)dst, (unsigned)N), c) : 0.f;
       } else {
         cs[c] = (c == 0 || 64 * c < S) && src_acc ? pol.coul(src, c) : 0.f;
         cd[c] = (c == 0 || 64 * c < S) && dst_acc ? pol.coul(dst, c) : 0.f;
@@ -563,7 +563,9 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
     }
 #pragma unroll
     for (int c = 0; c < C; ++c) {
-      if (src_acc && (src >> 6) == c) occ[c] &= ~(1ull << (src & 63));
+      // the source's bit is cleared whether it is an acceptor or not: bits of
+      // electrode sites are never set (a one-chunk kernel has S <= 64 sites)
+      if (C == 1 || (src >> 6) == c) occ[c] &= ~(1ull << (src & 63));
       if (dst_acc && (dst >> 6) == c) occ[c] |= 1ull << (dst & 63);
       se[c] = (se[c] - cs[c]) + cd[c];
       if (!src_acc && lane + 64 * c == src) eo[c] -= 1;
