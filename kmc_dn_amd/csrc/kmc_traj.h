@@ -153,9 +153,10 @@ struct TabPolicy {
     for (int c = 0; c < C; ++c) p.e[c] = col[64 * c];
     return p;
   }
-  __device__ __forceinline__ float pair(const Pre& p, int c, float diff) const {
-    if constexpr (BT) return min_f32(p.e[c].x, p.e[c].y + diff);
-    else return p.e[c].x + min_f32(0.f, diff + p.e[c].y);
+  // energy difference e~_src - e~_dst = sg * d (sg = +-1, exact)
+  __device__ __forceinline__ float pair(const Pre& p, int c, float d, float sg) const {
+    if constexpr (BT) return min_f32(p.e[c].x, fmaf(sg, d, p.e[c].y));
+    else return p.e[c].x + min_f32(0.f, fmaf(sg, d, p.e[c].y));
   }
   __device__ __forceinline__ float coul(int u, int c) const {
     if constexpr (BT) return ctab[u * S + lane + 64 * c];
@@ -210,12 +211,13 @@ struct PosPolicy {
     rinv = __builtin_amdgcn_rsqf(d2);
     return d2 * rinv;
   }
-  __device__ __forceinline__ float pair(const Pre& p, int c, float diff) const {
+  // energy difference e~_src - e~_dst = sg * dd (sg = +-1, exact)
+  __device__ __forceinline__ float pair(const Pre& p, int c, float dd, float sg) const {
     float rinv;
     const float d = dist_rinv(p.q, c, rinv);
     const float lt = fmaf(mkl, d, l0);
     const float cc = rinv * kcl[c] * p.q.w;
-    const float x = lt + min_f32(0.f, diff + cc);
+    const float x = lt + min_f32(0.f, fmaf(sg, dd, cc));
     // padding members and electrode-electrode pairs (an electrode with itself
     // included) have no rate: a wave-uniform lane mask selects them
     const uint64_t none = p.pad ? ~0ull : (p.el ? elm[c] : 0ull);
@@ -262,12 +264,13 @@ struct Team {
 };
 
 // Row sums over the uniform set U (bits of umask & own), lanes covering the
-// other role.  UDST: U = targets, lanes = sources; otherwise U = sources,
-// lanes = targets.
-template <int C, bool UDST, class Pol>
+// other role.  sg = +1: U = targets, lanes = sources (e~_src - e~_dst = v - su);
+// sg = -1: U = sources, lanes = targets.  One code path for both: the sign is a
+// scalar operand of the pair term's FMA.
+template <int C, class Pol>
 __device__ __forceinline__ void row_sums(const Pol& pol, int S, const uint64_t (&umask)[C],
                                          uint64_t own, const float (&v)[C],
-                                         const float (&se)[C], float (&R)[C]) {
+                                         const float (&se)[C], float sg, float (&R)[C]) {
   constexpr int U = 4;
 #pragma unroll
   for (int cu = 0; cu < C; ++cu) {
@@ -292,8 +295,7 @@ __device__ __forceinline__ void row_sums(const Pol& pol, int S, const uint64_t (
 #pragma unroll
         for (int c = 0; c < C; ++c) {
           if (c > 0 && 64 * c >= S) break;   // chunk past this layout's sites
-          const float diff = UDST ? (v[c] - su) : (su - v[c]);
-          R[c] += fast_exp2(pol.pair(pre[k], c, diff));
+          R[c] += fast_exp2(pol.pair(pre[k], c, v[c] - su, sg));
         }
       }
     }
@@ -409,12 +411,15 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
   int next_rc = next_multiple(h_begin);
   const int h_stop = status == ST_OK ? h_end : h_begin;   // a stopped trajectory stays
   int h = h_begin;
+  // uniforms of hops (hh & ~63) + lane: at the start and at every multiple of 64
+  auto refill = [&](int hh) {
+    double uc;
+    stream_uniforms(td.seed, td.hop0 + (uint64_t)((hh & ~(int64_t)63) + lane), uc, ue_lane);
+    lg_lane = fast_log2((float)(1.0 - uc)) * -0.69314718055994531f;
+  };
+  if (h < h_stop && (h & 63) != 0) refill(h);
   for (; h < h_stop; ++h) {
-    if (h == h_begin || (h & 63) == 0) {
-      double uc;
-      stream_uniforms(td.seed, td.hop0 + (uint64_t)((h & ~(int64_t)63) + lane), uc, ue_lane);
-      lg_lane = fast_log2((float)(1.0 - uc)) * -0.69314718055994531f;
-    }
+    if ((h & 63) == 0) refill(h);
     if (__builtin_expect(h == prehops, 0)) {
       time = 0.0;
 #pragma unroll
@@ -452,8 +457,9 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
     if constexpr (Pol::kElec) {
       if (elec) elec_sums<C, T>(pol, L.P, S, udst, team.wave, v, R);
     }
-    if (udst) row_sums<C, true>(pol, S, um, team.own(), v, se, R);
-    else row_sums<C, false>(pol, S, um, team.own(), v, se, R);
+    // +1: walked targets, e~_src - e~_dst = v - su; -1: walked sources
+    const float sgr = __int_as_float(udst ? (int)0x3f800000 : (int)0xbf800000);
+    row_sums<C>(pol, S, um, team.own(), v, se, sgr, R);
     team.reduce(R, (int)(h & 1), lane);
 
     // FP64 prefix over lane rows
@@ -514,8 +520,7 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
       if (c > 0 && 64 * c >= S) break;
       const bool inU = __builtin_amdgcn_inverse_ballot_w64(udst ? dstm[c] : srcm[c]);
       // udst ? s_sel - se : se - s_sel (the sign is exact)
-      const float diff = sg * (se[c] - s_sel);
-      float r = fast_exp2(pol.pair(pre, c, diff));
+      float r = fast_exp2(pol.pair(pre, c, se[c] - s_sel, sg));
       r = inU ? r : 0.f;
       const float cum = wave_scan_f32(r) + run;
       const uint64_t nz = ballot(r > 0.f);
