@@ -739,7 +739,9 @@ int setup_launch(Batch* B, const kmc_run_opts& o, int form, int W, int team, int
                  Fp32Launch& L) {
   Engine& E = eng();
   const int C = (smax + 63) / 64;
-  const bool rec = (o.record & 1) != 0;
+  // recording kernels: occupation time (bit 0) and traffic (bit 1); the others
+  // carry no per-hop test for either
+  const bool rec = (o.record & 3) != 0;
   const size_t lds_cap = (size_t)160 * 1024;
   BatchKernel k = nullptr;
   size_t lds = 0;

@@ -546,7 +546,7 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
         if (__builtin_amdgcn_inverse_ballot_w64(occ[c] & accm[c])) ot[c] += dt;
     }
     // net traffic, Go convention (simulation.go:310-311)
-    if (p.traffic && tally && lane == 0 && team.writer()) {
+    if (REC && p.traffic && tally && lane == 0 && team.writer()) {
       double* tr = p.traffic + (size_t)t * p.traffic_stride;
       tr[src * S + dst] += 1.0;
       tr[dst * S + src] -= 1.0;
@@ -571,10 +571,11 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
       // the source's bit is cleared whether it is an acceptor or not: bits of
       // electrode sites are never set (a one-chunk kernel has S <= 64 sites)
       if (C == 1 || (src >> 6) == c) occ[c] &= ~(1ull << (src & 63));
-      if (dst_acc && (dst >> 6) == c) occ[c] |= 1ull << (dst & 63);
+      if (dst_acc && (C == 1 || (dst >> 6) == c)) occ[c] |= 1ull << (dst & 63);
       se[c] = (se[c] - cs[c]) + cd[c];
-      if (!src_acc && lane + 64 * c == src) eo[c] -= 1;
-      if (!dst_acc && lane + 64 * c == dst) eo[c] += 1;
+      // tallies move on the lanes of both ends; only electrode lanes' are kept
+      if (lane + 64 * c == src) eo[c] -= 1;
+      if (lane + 64 * c == dst) eo[c] += 1;
     }
     if (REC && tally && p.post_hop_record) {
 #pragma unroll
