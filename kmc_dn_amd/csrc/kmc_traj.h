@@ -418,33 +418,46 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
     lg_lane = fast_log2((float)(1.0 - uc)) * -0.69314718055994531f;
   };
   if (h < h_stop && (h & 63) != 0) refill(h);
+  // the next hop with work besides the hop itself: a multiple of 64 (uniforms),
+  // the end of the prehops (tallies restart) or a multiple of K (recompute)
+  auto next_special = [&](int hh, bool inclusive) {
+    const int64_t r = inclusive ? ((int64_t)hh + 63) & ~(int64_t)63 : ((int64_t)(hh >> 6) + 1) << 6;
+    int n = (int)(r < INT32_MAX ? r : INT32_MAX);
+    if (next_rc < n) n = next_rc;
+    if ((inclusive ? prehops >= hh : prehops > hh) && prehops < n) n = prehops;
+    return n;
+  };
+  int special = next_special(h, true);
   for (; h < h_stop; ++h) {
-    if ((h & 63) == 0) refill(h);
-    if (__builtin_expect(h == prehops, 0)) {
-      time = 0.0;
+    if (__builtin_expect(h == special, 0)) {
+      if ((h & 63) == 0) refill(h);
+      if (h == prehops) {
+        time = 0.0;
 #pragma unroll
-      for (int c = 0; c < C; ++c) {
-        eo[c] = 0;
-        ot[c] = 0.0;
+        for (int c = 0; c < C; ++c) {
+          eo[c] = 0;
+          ot[c] = 0.0;
+        }
       }
-    }
-    if (h == next_rc) {
-      recompute_energies<C>(se, occ, L, es, lane);
-      next_rc = next_multiple(h);
+      if (h == next_rc) {
+        recompute_energies<C>(se, occ, L, es, lane);
+        next_rc = next_multiple(h);
+      }
+      special = next_special(h, false);
     }
 
     // roles of the lanes' sites, as wave-uniform lane masks (scalar ops only):
     // sources = occupied acceptors + electrodes, targets = empty acceptors + electrodes
     uint64_t srcm[C], dstm[C];
-    int nsrc = 0, ndst = 0;
+    int nocc = 0;
 #pragma unroll
     for (int c = 0; c < C; ++c) {
-      srcm[c] = (occ[c] & accm[c]) | elm[c];
-      dstm[c] = (~occ[c] & accm[c]) | elm[c];
-      nsrc += popcount64(srcm[c]);
-      ndst += popcount64(dstm[c]);
+      const uint64_t o = occ[c] & accm[c];
+      srcm[c] = o | elm[c];
+      dstm[c] = (accm[c] ^ o) | elm[c];
+      nocc += popcount64(o);
     }
-    const bool udst = ndst <= nsrc;   // walk the smaller set
+    const bool udst = N - nocc <= nocc;   // walk the smaller set (empty vs occupied)
     const float vfill = udst ? -INFINITY : INFINITY;
     float v[C], R[C];
     uint64_t um[C];
