@@ -303,10 +303,10 @@ __device__ __forceinline__ void row_sums(const Pol& pol, int S, const uint64_t (
 }
 
 // Closed-form electrode terms of the row sums (table policies).  v: the lane
-// sites' scaled energies in the lane role, -inf (udst) / +inf elsewhere, so
-// 2^(+-v) is 0 on lanes outside the role; electrode lanes have w = 0.  The
-// exponent is clamped to 126 (then x -+ V~ >= 26 and the min is 1 either way);
-// |V~| <= 100 keeps 2^-+V~ finite and nonzero, so the product is never NaN.
+// sites' scaled energies; the sums of lanes outside the role are discarded by
+// the caller, and electrode lanes have w = 0.  The exponent is clamped to 126
+// (then x -+ V~ >= 26 and the min is 1 either way); |V~| <= 100 keeps 2^-+V~
+// finite and nonzero, so the product is never NaN.
 // Electrodes go in blocks of four (all loads of a block issued together); rows
 // past P have w = 0 and factor 1.  A team splits the blocks: wave w takes the
 // blocks j = w (mod T).
@@ -458,15 +458,16 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
       nocc += popcount64(o);
     }
     const bool udst = N - nocc <= nocc;   // walk the smaller set (empty vs occupied)
-    const float vfill = udst ? -INFINITY : INFINITY;
-    float v[C], R[C];
+    // every lane sums over the walked members; the sums of lanes outside the
+    // other role (and past S) are discarded after the reduction
+    float R[C];
     uint64_t um[C];
 #pragma unroll
     for (int c = 0; c < C; ++c) {
       R[c] = 0.f;
-      v[c] = __builtin_amdgcn_inverse_ballot_w64(udst ? srcm[c] : dstm[c]) ? se[c] : vfill;
       um[c] = (udst ? dstm[c] : srcm[c]) & walk[c];
     }
+    const float (&v)[C] = se;
     if constexpr (Pol::kElec) {
       if (elec) elec_sums<C, T>(pol, L.P, S, udst, team.wave, v, R);
     }
@@ -474,6 +475,9 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
     const float sgr = __int_as_float(udst ? (int)0x3f800000 : (int)0xbf800000);
     row_sums<C>(pol, S, um, team.own(), v, se, sgr, R);
     team.reduce(R, (int)(h & 1), lane);
+#pragma unroll
+    for (int c = 0; c < C; ++c)
+      R[c] = __builtin_amdgcn_inverse_ballot_w64(udst ? srcm[c] : dstm[c]) ? R[c] : 0.f;
 
     // FP64 prefix over lane rows
     double incl[C];
@@ -492,10 +496,10 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
       break;
     }
     const float lg = readlane_f32(lg_lane, (int)(h & 63));
-    const double ue = readlane_f64(ue_lane, (int)(h & 63));
     // exponential waiting time -ln(1-u)/total, FP32 quotient into the FP64 clock
     const double dt = (double)(lg * __builtin_amdgcn_rcpf((float)total));
-    const double target = ue * total;
+    // event uniform x total, multiplied on every lane and read from this hop's
+    const double target = readlane_f64(ue_lane * total, (int)(h & 63));
 
     int sel = -1;
     double excl = 0.0;
