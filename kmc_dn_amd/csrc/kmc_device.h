@@ -117,13 +117,21 @@ __device__ __forceinline__ double wave_scan_f64(double x) {
   return x;
 }
 
+// The row broadcasts add in place with one DPP add each: rows outside the row
+// mask are not written and keep x, which is what x + 0 gives (the compiler's
+// move-then-add form needs a zeroed register and two more instructions).  The
+// s_nop gives the DPP read of x the two wait states it needs after the VALU write.
 __device__ __forceinline__ float wave_scan_f32(float x) {
   x += dpp_f32<DPP_ROW_SHR1, 0xF, true>(x);
   x += dpp_f32<DPP_ROW_SHR2, 0xF, true>(x);
   x += dpp_f32<DPP_ROW_SHR4, 0xF, true>(x);
   x += dpp_f32<DPP_ROW_SHR8, 0xF, true>(x);
-  x += dpp_f32<DPP_ROW_BCAST15, 0xA, false>(x);
-  x += dpp_f32<DPP_ROW_BCAST31, 0xC, false>(x);
+  asm volatile(
+      "s_nop 1\n\t"
+      "v_add_f32_dpp %0, %0, %0 row_bcast:15 row_mask:0xa bank_mask:0xf\n\t"
+      "s_nop 1\n\t"
+      "v_add_f32_dpp %0, %0, %0 row_bcast:31 row_mask:0xc bank_mask:0xf"
+      : "+v"(x));
   return x;
 }
 
