@@ -137,6 +137,8 @@ struct TabPolicy {
   // coul(N, c) is 0: column N of the c table lies in its zero padding, and column
   // N of the pair table is an electrode column (or the sentinel), c = 0
   static constexpr bool kZeroColumnN = true;
+  // the 64C entries before the table are -inf: column -1 (an exhausted walk) reads them
+  static constexpr bool kNegSentinel = true;
   const float2* tab;   // [S*S + 64C]: entry (s, u) at u*S + s; column S is all -inf
   const float* ctab;   // BT: [N*S + 64C] c at u*S + s
   const float* elw;    // [4*ceil(P/4)][64C] electrode weights of the lane sites
@@ -145,7 +147,7 @@ struct TabPolicy {
   struct Pre {
     float2 e[C];
   };
-  // u in [0, S): a member; u == S: padding (the -inf sentinel column at S*S)
+  // u in [0, S): a member; u == S or -1: padding (the -inf sentinel columns)
   __device__ __forceinline__ Pre prefetch(int u) const {
     Pre p;
     const float2* col = tab + u * S + lane;
@@ -168,6 +170,7 @@ template <int C>
 struct PosPolicy {
   static constexpr bool kElec = false;   // electrodes are walked as members
   static constexpr bool kZeroColumnN = false;
+  static constexpr bool kNegSentinel = false;
   const float* elw = nullptr;
   float* evc = nullptr;
   const float4* pos;   // [S] per wave: x, y, z, 1 for acceptors / 0 for electrodes
@@ -283,8 +286,11 @@ __device__ __forceinline__ void row_sums(const Pol& pol, int S, const uint64_t (
       for (int k = 0; k < U; ++k) {
         b[k] = lowest_bit(m);
         m = clear_bit(m, b[k]);
-        uu[k] = cu == 0 ? (int)min((unsigned)b[k], (unsigned)S)
-                        : (b[k] < 0 ? S : 64 * cu + b[k]);
+        if constexpr (Pol::kNegSentinel)   // column -1 is a -inf sentinel too
+          uu[k] = cu == 0 ? b[k] : (b[k] < 0 ? S : 64 * cu + b[k]);
+        else
+          uu[k] = cu == 0 ? (int)min((unsigned)b[k], (unsigned)S)
+                          : (b[k] < 0 ? S : 64 * cu + b[k]);
       }
       typename Pol::Pre pre[U];
 #pragma unroll
