@@ -312,33 +312,39 @@ __device__ __forceinline__ void row_sums(const Pol& pol, int S, const uint64_t (
 // sites' scaled energies; the sums of lanes outside the role are discarded by
 // the caller, and electrode lanes have w = 0.  The exponent is clamped to 126
 // (then x -+ V~ >= 26 and the min is 1 either way); |V~| <= 100 keeps 2^-+V~
-// finite and nonzero, so the product is never NaN.
+// finite and nonzero, so the product is never NaN, and min(1, product) of a
+// non-negative product is the multiply's own clamp to [0, 1].
 // Electrodes go in blocks of four (all loads of a block issued together); rows
 // past P have w = 0 and factor 1.  A team splits the blocks: wave w takes the
-// blocks j = w (mod T).
+// blocks j = w (mod T).  The blocks are unrolled to the PMAX / 4 a layout can
+// have, so every load takes an immediate offset from one base per hop; a
+// wave-uniform branch skips the blocks past P.
 template <int C, int T, class Pol>
 __device__ __forceinline__ void elec_sums(const Pol& pol, int P, int S, bool udst, int wave,
                                           const float (&v)[C], float (&R)[C]) {
   float ex[C];
 #pragma unroll
   for (int c = 0; c < C; ++c) ex[c] = fast_exp2(min_f32(udst ? v[c] : -v[c], 126.f));
-  const float* k = pol.evc + (udst ? PMAX : 0);
-  const float* w = pol.elw + pol.lane;
+  const float* k = pol.evc + (udst ? PMAX : 0) + 4 * wave;
+  const float* w = pol.elw + pol.lane + 4 * wave * 64 * C;
   const int nb = (P + 3) >> 2;
-  for (int j = wave; j < nb; j += T) {
-    const float4 k4 = *reinterpret_cast<const float4*>(k + 4 * j);
+  constexpr int kBlocks = (PMAX / 4 + T - 1) / T;
+#pragma unroll
+  for (int i = 0; i < kBlocks; ++i) {
+    if (wave + T * i >= nb) break;
+    const float4 k4 = *reinterpret_cast<const float4*>(k + 4 * T * i);
     const float kq[4] = {k4.x, k4.y, k4.z, k4.w};
     float wq[4][C];
 #pragma unroll
     for (int q = 0; q < 4; ++q)
 #pragma unroll
-      for (int c = 0; c < C; ++c) wq[q][c] = w[(4 * j + q) * 64 * C + 64 * c];
+      for (int c = 0; c < C; ++c) wq[q][c] = w[(4 * T * i + q) * 64 * C + 64 * c];
 #pragma unroll
     for (int q = 0; q < 4; ++q)
 #pragma unroll
       for (int c = 0; c < C; ++c) {
         if (c > 0 && 64 * c >= S) break;
-        R[c] = fmaf(wq[q][c], min_f32(1.f, ex[c] * kq[q]), R[c]);
+        R[c] = fmaf(wq[q][c], __builtin_amdgcn_fmed3f(ex[c] * kq[q], 0.f, 1.f), R[c]);
       }
   }
 }
@@ -596,9 +602,15 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
       if (C == 1 || (src >> 6) == c) occ[c] &= ~(1ull << (src & 63));
       if (dst_acc && (C == 1 || (dst >> 6) == c)) occ[c] |= 1ull << (dst & 63);
       se[c] = (se[c] - cs[c]) + cd[c];
-      // tallies move on the lanes of both ends; only electrode lanes' are kept
-      if (lane + 64 * c == src) eo[c] -= 1;
-      if (lane + 64 * c == dst) eo[c] += 1;
+    }
+    // electrode tallies (a wave-uniform branch: most hops join two acceptors);
+    // they move on the lanes of both ends; only electrode lanes' are kept
+    if (__builtin_expect(!src_acc || !dst_acc, 0)) {
+#pragma unroll
+      for (int c = 0; c < C; ++c) {
+        if (lane + 64 * c == src) eo[c] -= 1;
+        if (lane + 64 * c == dst) eo[c] += 1;
+      }
     }
     if (REC && tally && p.post_hop_record) {
 #pragma unroll
