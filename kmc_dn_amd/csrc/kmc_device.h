@@ -105,6 +105,16 @@ __device__ __forceinline__ double dpp_f64(double x) {
   return __longlong_as_double((long long)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo));
 }
 
+// DPP move that writes only the rows of ROW_MASK; the other rows keep `old`.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_f64_rows(double old, double x) {
+  const uint64_t b = __double_as_longlong(x), o = __double_as_longlong(old);
+  const int lo = __builtin_amdgcn_update_dpp((int)(uint32_t)o, (int)(uint32_t)b, CTRL, ROW_MASK, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp((int)(uint32_t)(o >> 32), (int)(uint32_t)(b >> 32), CTRL,
+                                             ROW_MASK, 0xF, false);
+  return __longlong_as_double((long long)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo));
+}
+
 // Inclusive scan over the 64 lanes of a wave (Hillis-Steele inside each row of
 // 16 lanes, then row broadcasts).  Every lane must be active.
 __device__ __forceinline__ double wave_scan_f64(double x) {
@@ -114,6 +124,22 @@ __device__ __forceinline__ double wave_scan_f64(double x) {
   x += dpp_f64<DPP_ROW_SHR8, 0xF, true>(x);
   x += dpp_f64<DPP_ROW_BCAST15, 0xA, false>(x);
   x += dpp_f64<DPP_ROW_BCAST31, 0xC, false>(x);
+  return x;
+}
+
+// The same scan for loops: the row broadcasts land in two carried registers
+// whose rows outside each broadcast's row mask are zero (rows 0 and 2 of b15,
+// rows 0 and 1 of b31) and are never written, so no per-call zeroing is needed.
+// Both start as 0.0 and are passed to every call.
+__device__ __forceinline__ double wave_scan_f64(double x, double& b15, double& b31) {
+  x += dpp_f64<DPP_ROW_SHR1, 0xF, true>(x);
+  x += dpp_f64<DPP_ROW_SHR2, 0xF, true>(x);
+  x += dpp_f64<DPP_ROW_SHR4, 0xF, true>(x);
+  x += dpp_f64<DPP_ROW_SHR8, 0xF, true>(x);
+  b15 = dpp_f64_rows<DPP_ROW_BCAST15, 0xA>(b15, x);
+  x += b15;
+  b31 = dpp_f64_rows<DPP_ROW_BCAST31, 0xC>(b31, x);
+  x += b31;
   return x;
 }
 

@@ -349,6 +349,14 @@ __device__ __forceinline__ void elec_sums(const Pol& pol, int P, int S, bool uds
   }
 }
 
+// (float)x of lane l: converted on every lane, then one readlane (the compiler
+// would otherwise read both halves of x and convert the scalar pair)
+__device__ __forceinline__ float readlane_as_f32(double x, int l) {
+  float f = (float)x;
+  asm("" : "+v"(f));
+  return readlane_f32(f, l);
+}
+
 template <int C>
 __device__ __forceinline__ float pick_lane_value(const float (&x)[C], int site) {
   const int cs = site >> 6, ls = site & 63;
@@ -440,6 +448,7 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
     return n;
   };
   int special = next_special(h, true);
+  double b15 = 0.0, b31 = 0.0;   // row-broadcast registers of the FP64 scan
   for (; h < h_stop; ++h) {
     if (__builtin_expect(h == special, 0)) {
       if ((h & 63) == 0) refill(h);
@@ -500,7 +509,8 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
         incl[c] = total;
         continue;
       }
-      incl[c] = c == 0 ? wave_scan_f64((double)R[c]) : wave_scan_f64((double)R[c]) + total;
+      incl[c] = c == 0 ? wave_scan_f64((double)R[c], b15, b31)
+                       : wave_scan_f64((double)R[c], b15, b31) + total;
       total = readlane_f64(incl[c], 63);
     }
     if (!(total > 0.0) || !(total < INFINITY)) {
@@ -513,15 +523,17 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
     // event uniform x total, multiplied on every lane and read from this hop's
     const double target = readlane_f64(ue_lane * total, (int)(h & 63));
 
+    // the selected lane l and the remainder rr = target - (incl - R) at l, formed
+    // on every lane (FP64 difference, rounded to FP32) and read from l
     int sel = -1;
-    double excl = 0.0;
+    float rr = 0.f;
 #pragma unroll
     for (int c = 0; c < C; ++c) {
       const uint64_t m = ballot(incl[c] > target);
       if (sel < 0 && m) {
         const int l = __builtin_ctzll(m);
         sel = 64 * c + l;
-        excl = readlane_f64(incl[c] - (double)R[c], l);
+        rr = readlane_as_f32(target - (incl[c] - (double)R[c]), l);
       }
     }
     if (sel < 0) {   // target rounded onto the total: last lane with a nonzero row
@@ -531,11 +543,10 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
         if (m) {
           const int l = 63 - __builtin_clzll(m);
           sel = 64 * c + l;
-          excl = readlane_f64(incl[c] - (double)R[c], l);
+          rr = readlane_as_f32(target - (incl[c] - (double)R[c]), l);
         }
       }
     }
-    const float rr = (float)(target - excl);
 
     // partner of `sel`: lane-parallel pass over its row, in site order
     const float s_sel = pick_lane_value<C>(se, sel);
@@ -553,7 +564,7 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
       r = inU ? r : 0.f;
       const float cum = wave_scan_f32(r) + run;
       const uint64_t nz = ballot(r > 0.f);
-      const uint64_t hit = ballot(r > 0.f && cum > rr);
+      const uint64_t hit = nz & ballot(cum > rr);
       if (pick < 0) {
         if (nz) last_nz = 64 * c + 63 - __builtin_clzll(nz);
         if (hit) pick = 64 * c + __builtin_ctzll(hit);
