@@ -266,44 +266,51 @@ struct Team {
   }
 };
 
+// K members of the walked set (the lowest K bits of m, all present; K <= the
+// number of bits left): their columns are loaded together, then their terms added.
+template <int K, int C, class Pol>
+__device__ __forceinline__ void row_block(const Pol& pol, int S, int cu, uint64_t& m,
+                                          const float (&v)[C], const float (&se)[C], float sg,
+                                          float (&R)[C]) {
+  int b[K];
+  typename Pol::Pre pre[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    b[k] = lowest_bit(m);
+    m = clear_bit(m, b[k]);
+  }
+#pragma unroll
+  for (int k = 0; k < K; ++k) pre[k] = pol.prefetch(64 * cu + b[k]);
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const float su = readlane_f32(se[cu], b[k]);
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      if (c > 0 && 64 * c >= S) break;   // chunk past this layout's sites
+      R[c] += fast_exp2(pol.pair(pre[k], c, v[c] - su, sg));
+    }
+  }
+}
+
 // Row sums over the uniform set U (bits of umask & own), lanes covering the
 // other role.  sg = +1: U = targets, lanes = sources (e~_src - e~_dst = v - su);
 // sg = -1: U = sources, lanes = targets.  One code path for both: the sign is a
-// scalar operand of the pair term's FMA.
+// scalar operand of the pair term's FMA.  Members go in blocks of four; the last
+// block takes exactly the members left (a scalar branch on their count), so no
+// lane computes a padding term.
 template <int C, class Pol>
 __device__ __forceinline__ void row_sums(const Pol& pol, int S, const uint64_t (&umask)[C],
                                          uint64_t own, const float (&v)[C],
                                          const float (&se)[C], float sg, float (&R)[C]) {
-  constexpr int U = 4;
 #pragma unroll
   for (int cu = 0; cu < C; ++cu) {
     uint64_t m = umask[cu] & own;
     while (m) {
-      // branch-free: an exhausted mask yields lane -1, i.e. member S (padding),
-      // and stays empty
-      int b[U], uu[U];
-#pragma unroll
-      for (int k = 0; k < U; ++k) {
-        b[k] = lowest_bit(m);
-        m = clear_bit(m, b[k]);
-        if constexpr (Pol::kNegSentinel)   // column -1 is a -inf sentinel too
-          uu[k] = cu == 0 ? b[k] : (b[k] < 0 ? S : 64 * cu + b[k]);
-        else
-          uu[k] = cu == 0 ? (int)min((unsigned)b[k], (unsigned)S)
-                          : (b[k] < 0 ? S : 64 * cu + b[k]);
-      }
-      typename Pol::Pre pre[U];
-#pragma unroll
-      for (int k = 0; k < U; ++k) pre[k] = pol.prefetch(uu[k]);
-#pragma unroll
-      for (int k = 0; k < U; ++k) {
-        const float su = readlane_f32(se[cu], b[k] & 63);
-#pragma unroll
-        for (int c = 0; c < C; ++c) {
-          if (c > 0 && 64 * c >= S) break;   // chunk past this layout's sites
-          R[c] += fast_exp2(pol.pair(pre[k], c, v[c] - su, sg));
-        }
-      }
+      const int n = popcount64(m);
+      if (n >= 4) row_block<4>(pol, S, cu, m, v, se, sg, R);
+      else if (n == 3) row_block<3>(pol, S, cu, m, v, se, sg, R);
+      else if (n == 2) row_block<2>(pol, S, cu, m, v, se, sg, R);
+      else row_block<1>(pol, S, cu, m, v, se, sg, R);
     }
   }
 }
