@@ -44,7 +44,7 @@ def table_lds_bytes(form: int, S: int, N: int, waves: int = 8) -> int:
     waves' electrode factors, the electrode weights of up to 16 electrodes, the
     pair table and, for form 1, the c table (kmc_batch.hip batch_lds_bytes)."""
     C = (S + 63) // 64
-    b = waves * 2 * 16 * 4 + 16 * 64 * C * 4 + (S * S + 2 * 64 * C) * 8
+    b = waves * 2 * 16 * 4 + 16 * 64 * C * 4 + (S * S + 64 * C) * 8
     if form == 1:
         b += (N * S + 64 * C) * 4
     return b

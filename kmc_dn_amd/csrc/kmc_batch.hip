@@ -32,12 +32,10 @@ __global__ void __launch_bounds__(64 * W) kmc_batch_fp32_kernel(BatchParams p) {
     if (g >= p.ngroups) break;
     const GroupDev gd = p.groups[g];
     const LayoutDev L = p.layouts[gd.layout];
-    // 64C -inf entries, then the pair table: column -1 reads the sentinel
-    float2* tab = reinterpret_cast<float2*>(elw + elw_lds_floats(L.P, C)) + 64 * C;
+    float2* tab = reinterpret_cast<float2*>(elw + elw_lds_floats(L.P, C));
     float* ctab = reinterpret_cast<float*>(tab + L.S * L.S + 64 * C);
     if (gd.layout != __builtin_amdgcn_readfirstlane(s_layout)) {
       load_elw<C>(elw, L, threadIdx.x, 64 * W);
-      for (int k = threadIdx.x; k < 64 * C; k += 64 * W) tab[k - 64 * C] = make_float2(-INFINITY, -INFINITY);
       const int n = L.S * L.S + 64 * C;
       const float2* src = BT ? L.tabB : L.tab32;
       for (int k = threadIdx.x; k < n; k += 64 * W) tab[k] = src[k];
@@ -114,7 +112,7 @@ __global__ void kmc_eset_scatter_kernel(const EsetDev* esets, const int32_t* ids
 size_t batch_lds_bytes(bool bt, int C, int W, int S, int N, int P) {
   size_t b = (size_t)W * 2 * PMAX * sizeof(float);
   b += (size_t)elw_lds_floats(P, C) * sizeof(float);
-  b += ((size_t)S * S + 2 * 64 * C) * sizeof(float2);   // sentinel before, padding after
+  b += ((size_t)S * S + 64 * C) * sizeof(float2);
   if (bt) b += ((size_t)N * S + 64 * C) * sizeof(float);
   return b;
 }

@@ -44,12 +44,10 @@ __global__ void __launch_bounds__(64 * T) kmc_team_kernel(BatchParams p) {
       constexpr bool BT = FORM == 1;
       float* evc = reinterpret_cast<float*>(rest) + wave * 2 * PMAX;
       float* elw = reinterpret_cast<float*>(rest) + T * 2 * PMAX;
-      // 64C -inf entries, then the pair table: column -1 reads the sentinel
-      float2* tab = reinterpret_cast<float2*>(elw + elw_lds_floats(L.P, C)) + 64 * C;
+      float2* tab = reinterpret_cast<float2*>(elw + elw_lds_floats(L.P, C));
       float* ctab = reinterpret_cast<float*>(tab + L.S * L.S + 64 * C);
       if (gd.layout != __builtin_amdgcn_readfirstlane(s_layout)) {
         load_elw<C>(elw, L, threadIdx.x, 64 * T);
-        for (int k = threadIdx.x; k < 64 * C; k += 64 * T) tab[k - 64 * C] = make_float2(-INFINITY, -INFINITY);
         const int n = L.S * L.S + 64 * C;
         const float2* src = BT ? L.tabB : L.tab32;
         for (int k = threadIdx.x; k < n; k += 64 * T) tab[k] = src[k];
@@ -84,7 +82,7 @@ size_t team_lds_bytes(int form, int C, int T, int S, int N, int P) {
   if (form == 3) return b + (size_t)64 * C * sizeof(float4);
   b += (size_t)T * 2 * PMAX * sizeof(float);
   b += (size_t)elw_lds_floats(P, C) * sizeof(float);
-  b += ((size_t)S * S + 2 * 64 * C) * sizeof(float2);   // sentinel before, padding after
+  b += ((size_t)S * S + 64 * C) * sizeof(float2);
   if (form == 1) b += ((size_t)N * S + 64 * C) * sizeof(float);
   return b;
 }

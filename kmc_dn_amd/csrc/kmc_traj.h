@@ -125,8 +125,7 @@ __device__ __forceinline__ float min_f32(float a, float b) {
 }
 
 // ---------------------------------------------------------------------------
-// Pair-term policies.  For a uniform member u of the walked set (u < 0: none)
-// and the lane sites s = lane + 64c, pair(pre(u), c, diff) is log2 of the rate
+// Pair-term policies.  For a uniform member u of the walked set and the lane sites s = lane + 64c, pair(pre(u), c, diff) is log2 of the rate
 // of the pair (src, dst) with e~_src - e~_dst = diff; coul(u, c) is c(s, u) for
 // the rank-1 energy update (u an acceptor).  prefetch(u) issues the loads that
 // pair() needs, so four members can be in flight together.
@@ -135,11 +134,12 @@ template <int C, bool BT>
 struct TabPolicy {
   static constexpr bool kElec = true;   // electrodes in closed form (elec_sums)
   // coul(N, c) is 0: column N of the c table lies in its zero padding, and column
-  // N of the pair table is an electrode column (or the sentinel), c = 0
+  // N of the pair table is an electrode column (or, with no electrodes, the
+  // padding), c = 0
   static constexpr bool kZeroColumnN = true;
-  // the 64C entries before the table are -inf: column -1 (an exhausted walk) reads them
-  static constexpr bool kNegSentinel = true;
-  const float2* tab;   // [S*S + 64C]: entry (s, u) at u*S + s; column S is all -inf
+  // [S*S + 64C]: entry (s, u) at u*S + s; the 64C padding entries (lt = -inf)
+  // take the reads of the lanes past S in column S - 1
+  const float2* tab;
   const float* ctab;   // BT: [N*S + 64C] c at u*S + s
   const float* elw;    // [4*ceil(P/4)][64C] electrode weights of the lane sites
   float* evc;          // this wave's [2][PMAX]: 2^V~_e, then 2^-V~_e
@@ -147,7 +147,7 @@ struct TabPolicy {
   struct Pre {
     float2 e[C];
   };
-  // u in [0, S): a member; u == S or -1: padding (the -inf sentinel columns)
+  // u in [0, S)
   __device__ __forceinline__ Pre prefetch(int u) const {
     Pre p;
     const float2* col = tab + u * S + lane;
@@ -170,7 +170,6 @@ template <int C>
 struct PosPolicy {
   static constexpr bool kElec = false;   // electrodes are walked as members
   static constexpr bool kZeroColumnN = false;
-  static constexpr bool kNegSentinel = false;
   const float* elw = nullptr;
   float* evc = nullptr;
   const float4* pos;   // [S] per wave: x, y, z, 1 for acceptors / 0 for electrodes
@@ -180,7 +179,7 @@ struct PosPolicy {
   int lane, N, S;
   struct Pre {
     float4 q;
-    bool pad, el;      // wave-uniform: no member / the member is an electrode
+    bool el;           // wave-uniform: the member is an electrode
   };
   __device__ __forceinline__ void init(const float4* p, const LayoutDev& L, int ln) {
     pos = p;
@@ -200,11 +199,10 @@ struct PosPolicy {
       elm[c] = chunk_mask(L.N, L.S, c);
     }
   }
-  // u in [0, S): a member; u == S: padding
+  // u in [0, S)
   __device__ __forceinline__ Pre prefetch(int u) const {
     Pre p;
-    p.pad = u >= S;
-    p.q = pos[p.pad ? 0 : u];
+    p.q = pos[u];
     p.el = u >= N;
     return p;
   }
@@ -221,10 +219,9 @@ struct PosPolicy {
     const float lt = fmaf(mkl, d, l0);
     const float cc = rinv * kcl[c] * p.q.w;
     const float x = lt + min_f32(0.f, fmaf(sg, dd, cc));
-    // padding members and electrode-electrode pairs (an electrode with itself
-    // included) have no rate: a wave-uniform lane mask selects them
-    const uint64_t none = p.pad ? ~0ull : (p.el ? elm[c] : 0ull);
-    return __builtin_amdgcn_inverse_ballot_w64(none) ? -INFINITY : x;
+    // electrode-electrode pairs (an electrode with itself included) have no
+    // rate: a wave-uniform lane mask selects them
+    return __builtin_amdgcn_inverse_ballot_w64(p.el ? elm[c] : 0ull) ? -INFINITY : x;
   }
   __device__ __forceinline__ float coul(int u, int c) const {
     const float4 q = pos[u];
