@@ -48,7 +48,7 @@ __global__ void __launch_bounds__(64 * W) kmc_batch_fp32_kernel(BatchParams p) {
     }
     __syncthreads();
     if (wave < gd.count) {
-      const TabPolicy<C, BT> pol{tab, ctab, elw, evc, L.S, lane};
+      const TabPolicy<C, BT> pol = TabPolicy<C, BT>::make(tab, ctab, elw, evc, L.S, lane);
       run_trajectory<C, REC>(p, gd.first + wave, L, pol, lane);
     }
   }

@@ -59,7 +59,7 @@ __global__ void __launch_bounds__(64 * T) kmc_team_kernel(BatchParams p) {
         if (threadIdx.x == 0) s_layout = gd.layout;
       }
       __syncthreads();
-      const TabPolicy<C, BT> pol{tab, ctab, elw, evc, L.S, lane};
+      const TabPolicy<C, BT> pol = TabPolicy<C, BT>::make(tab, ctab, elw, evc, L.S, lane);
       for (int i = 0; i < gd.count; ++i) run_trajectory<C, REC>(p, gd.first + i, L, pol, lane, team);
     }
   }

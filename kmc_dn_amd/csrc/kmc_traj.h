@@ -125,8 +125,9 @@ __device__ __forceinline__ float min_f32(float a, float b) {
 }
 
 // ---------------------------------------------------------------------------
-// Pair-term policies.  For a uniform member u of the walked set and the lane sites s = lane + 64c, pair(pre(u), c, diff) is log2 of the rate
-// of the pair (src, dst) with e~_src - e~_dst = diff; coul(u, c) is c(s, u) for
+// Pair-term policies.  For a uniform member u of the walked set and the lane
+// sites s = lane + 64c, pair(pre(u), c, diff) is log2 of the rate of the pair
+// (src, dst) with e~_src - e~_dst = diff; coul(u, c) is c(s, u) for
 // the rank-1 energy update (u an acceptor).  prefetch(u) issues the loads that
 // pair() needs, so four members can be in flight together.
 // ---------------------------------------------------------------------------
@@ -144,15 +145,29 @@ struct TabPolicy {
   const float* elw;    // [4*ceil(P/4)][64C] electrode weights of the lane sites
   float* evc;          // this wave's [2][PMAX]: 2^V~_e, then 2^-V~_e
   int S, lane;
+  // column strides in bytes (S * 8, S * 4), held in vector registers so that a
+  // member's column address is one v_mad_u32_u24 of its scalar index
+  int stride2, stride1;
+  __device__ __forceinline__ static TabPolicy make(const float2* tab, const float* ctab,
+                                                   const float* elw, float* evc, int S, int lane) {
+    TabPolicy p{tab, ctab, elw, evc, S, lane, S * 8, S * 4};
+    asm("" : "+v"(p.stride2), "+v"(p.stride1));
+    return p;
+  }
+  // the lane's entry of column u: tab + lane + u*S
+  __device__ __forceinline__ const float2* col(int u) const {
+    return reinterpret_cast<const float2*>(reinterpret_cast<const char*>(tab + lane) +
+                                           __umul24((unsigned)u, (unsigned)stride2));
+  }
   struct Pre {
     float2 e[C];
   };
   // u in [0, S)
   __device__ __forceinline__ Pre prefetch(int u) const {
     Pre p;
-    const float2* col = tab + u * S + lane;
+    const float2* cl = col(u);
 #pragma unroll
-    for (int c = 0; c < C; ++c) p.e[c] = col[64 * c];
+    for (int c = 0; c < C; ++c) p.e[c] = cl[64 * c];
     return p;
   }
   // energy difference e~_src - e~_dst = sg * d (sg = +-1, exact)
@@ -161,8 +176,10 @@ struct TabPolicy {
     else return p.e[c].x + min_f32(0.f, fmaf(sg, d, p.e[c].y));
   }
   __device__ __forceinline__ float coul(int u, int c) const {
-    if constexpr (BT) return ctab[u * S + lane + 64 * c];
-    else return tab[u * S + lane + 64 * c].y;
+    if constexpr (BT)
+      return reinterpret_cast<const float*>(reinterpret_cast<const char*>(ctab + lane) +
+                                            __umul24((unsigned)u, (unsigned)stride1))[64 * c];
+    else return col(u)[64 * c].y;
   }
 };
 
