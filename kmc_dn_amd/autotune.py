@@ -35,6 +35,13 @@ LDS_BYTES = 160 * 1024
 # concurrent bucket launches: HIP gives a process 4 hardware queues by default and
 # the default stream (host copies) holds one of them
 MAX_LAUNCHES = 3
+# hop latency of a solo trajectory inside a loaded batch over its latency alone:
+# 1.19-1.21 measured for positions-form trajectories at S = 130-131 of the
+# 2048-network uneven batch (7.1-7.25 against 5.97 us/hop)
+BATCH_SLOWDOWN = 1.25
+# least ratio of the best solo latency to the team latency for a team entry
+# (measured alone: 1.14 at S = 64, 1.49 at S = 128, 1.52 at S = 132)
+TEAM_MIN_GAIN = 1.2
 CACHE_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "autotune_cache.json")
 
 
@@ -197,11 +204,17 @@ def plan_for(trajs: np.ndarray, sizes: dict, has_pos: bool = True, edges=BUCKET_
     for edge, form, w, _ in solo:
         b = edges.index(edge)
         lt = lat.get(edge)
-        if lt and lt["team"] > 1:
+        # a team entry only where a team is clearly faster alone: a marginal gain
+        # (small networks) does not pay for the launch slot it takes, and a
+        # fourth entry forces the larger sizes' team entries to merge into one
+        # kernel sized for the largest table
+        solo_alone = min(lt["solo"].values()) if lt and lt["solo"] else None
+        if lt and lt["team"] > 1 and solo_alone and lt["lat"] * TEAM_MIN_GAIN <= solo_alone:
             if team_min_hops is not None:
                 thr = team_min_hops.get(edge)
             else:
-                solo_lat = lt["solo"].get(str(form), lt["lat"])
+                # a solo trajectory inside the loaded batch runs slower than alone
+                solo_lat = lt["solo"].get(str(form), lt["lat"]) * BATCH_SLOWDOWN
                 thr = int(math.ceil(t_thr / solo_lat)) if t_thr > 0 else None
                 # only worth a bucket when a trajectory of this size exceeds it
                 sel = bucket == b

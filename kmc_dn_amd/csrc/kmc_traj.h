@@ -225,7 +225,9 @@ struct PosPolicy {
   }
   __device__ __forceinline__ float dist_rinv(const float4& q, int c, float& rinv) const {
     const float dx = px[c] - q.x, dy = py[c] - q.y, dz = pz[c] - q.z;
-    const float d2 = fmaxf(dx * dx + dy * dy + dz * dz, 1e-30f);
+    // explicit FMAs: the same bits in every instantiation, whatever the
+    // compiler would contract around it
+    const float d2 = fmaxf(fmaf(dz, dz, fmaf(dy, dy, dx * dx)), 1e-30f);
     rinv = __builtin_amdgcn_rsqf(d2);
     return d2 * rinv;
   }
