@@ -430,6 +430,11 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
   // electrodes in closed form when the policy has the weights and every scaled
   // voltage satisfies |V~| <= 100; otherwise they are walked as members
   bool elec = false;
+  // Hops on which every lane of the other role has all its electrode terms
+  // saturated (x >= max V~ side) or all below 1 take per-lane sums fixed for the
+  // trajectory: sum_e w, or 2^x sum_e w 2^-+V~ (x = +-v as in elec_sums).
+  float esum_w[C], esum_dst[C], esum_src[C];   // sum w, sum w 2^-V~, sum w 2^V~
+  float vt_max = 0.f, vt_min = 0.f;
   if constexpr (Pol::kElec) {
     const float vt = lane < L.P ? (float)(es.V[lane] * (double)L.scale2) : 0.f;
     elec = L.P > 0 && L.elw_ok && ballot(!(fabsf(vt) <= 100.f)) == 0;
@@ -438,6 +443,25 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
       pol.evc[PMAX + lane] = fast_exp2(-vt);
     }
     wave_lds_sync();
+#pragma unroll
+    for (int c = 0; c < C; ++c) esum_w[c] = esum_dst[c] = esum_src[c] = 0.f;
+    if (elec && T == 1) {
+      vt_max = -INFINITY;
+      vt_min = INFINITY;
+      for (int e = 0; e < L.P; ++e) {
+        const float ve = readlane_f32(vt, e);
+        vt_max = fmaxf(vt_max, ve);
+        vt_min = fminf(vt_min, ve);
+        const float ks = pol.evc[e], kd = pol.evc[PMAX + e];
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+          const float w = pol.elw[e * 64 * C + 64 * c + lane];
+          esum_w[c] += w;
+          esum_dst[c] = fmaf(w, kd, esum_dst[c]);
+          esum_src[c] = fmaf(w, ks, esum_src[c]);
+        }
+      }
+    }
   }
   uint64_t walk[C];   // sites that may be walked: acceptors, plus electrodes unless elec
 #pragma unroll
@@ -513,7 +537,32 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
     }
     const float (&v)[C] = se;
     if constexpr (Pol::kElec) {
-      if (elec) elec_sums<C, T>(pol, L.P, S, udst, team.wave, v, R);
+      if (elec && T > 1) {
+        // teams run large networks, where some lane is nearly always in the
+        // window: no test
+        elec_sums<C, T>(pol, L.P, S, udst, team.wave, v, R);
+      } else if (elec) {
+        // the lanes whose electrode terms are neither all saturated nor all
+        // below 1 (acceptors of the other role); none: the per-lane sums
+        const float thi = udst ? vt_max : -vt_min, tlo = udst ? vt_min : -vt_max;
+        float x[C];
+        uint64_t hi[C], mid = 0;
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+          x[c] = min_f32(udst ? v[c] : -v[c], 126.f);
+          hi[c] = ballot(x[c] >= thi);
+          const uint64_t lo = ballot(x[c] < tlo);
+          mid |= (udst ? srcm[c] : dstm[c]) & accm[c] & ~(hi[c] | lo);
+        }
+        if (mid == 0) {
+#pragma unroll
+          for (int c = 0; c < C; ++c)
+            R[c] = __builtin_amdgcn_inverse_ballot_w64(hi[c])
+                       ? esum_w[c] : fast_exp2(x[c]) * (udst ? esum_dst[c] : esum_src[c]);
+        } else {
+          elec_sums<C, T>(pol, L.P, S, udst, team.wave, v, R);
+        }
+      }
     }
     // +1: walked targets, e~_src - e~_dst = v - su; -1: walked sources
     const float sgr = __int_as_float(udst ? (int)0x3f800000 : (int)0xbf800000);
