@@ -37,15 +37,15 @@ METRIC = "hops/sec, 4096-traj N=30 ensemble vs plain; p50/p99 latency of served 
 # divides by the LDS read peak measured in the same run (ds_read_b128 stream probe).
 LDS_PEAK_SPEC_GBS = 256 * 256 * 2.4
 # HBM bytes per launch of the batched kernel on the default workload (4096 x 1e5 hops,
-# N=30/P=8, auto table form and workgroup size): rocprofv3 --pmc FETCH_SIZE 391.062 KiB
+# N=30/P=8, auto table form and workgroup size): rocprofv3 --pmc FETCH_SIZE 390.312 KiB
 # and WRITE_SIZE 640 KiB per dispatch, scaled by the copy-kernel calibration factors
-# 2.0 (FETCH) and 1.0 (WRITE): (2 * 391.062 + 640) * 1024 bytes.
-HBM_TRAFFIC_BYTES = 1_456_255
+# 2.0 (FETCH) and 1.0 (WRITE): (2 * 390.312 + 640) * 1024 bytes.
+HBM_TRAFFIC_BYTES = 1_454_719
 # VALU instructions per hop and wave of the batched kernel on the default workload
-# (rocprofv3 --pmc SQ_INSTS_VALU: 4.726e10 per dispatch of 4.096e8 hops), and the VALU
+# (rocprofv3 --pmc SQ_INSTS_VALU: 4.288e10 per dispatch of 4.096e8 hops), and the VALU
 # issue peak: one wave64 instruction per 4 cycles on each of a CU's 4 SIMDs (16 lanes),
 # i.e. one per CU and cycle, 256 CUs at 2.4 GHz.
-VALU_INSTS_PER_HOP = 115.4
+VALU_INSTS_PER_HOP = 104.7
 VALU_ISSUE_PEAK = 256 * 2.4e9
 
 
@@ -313,7 +313,7 @@ def main() -> int:
                                 if default_workload else None),
             "note": "algorithmic LDS bytes of the dense formulation / kernel time; peak = "
                     "ds_read_b128 stream measured in this run; the kernel is bound by the VALU "
-                    "its 4 waves per SIMD share (VALUBusy 75%, LdsUtil 22%; 115.4 VALU and 80.6 scalar "
+                    "its 4 waves per SIMD share (VALUBusy 73%, LdsUtil 9%; 104.7 VALU and 81.7 scalar "
                     "instructions per hop): valu_issue_frac = profiled VALU "
                     "instructions per hop x hops / kernel time / (1 per CU and cycle)",
         },
