@@ -17,15 +17,19 @@
 //            computed per pair, ~2.5x the VALU work but no table, so waves of
 //            different layouts fill a CU independently.
 //  * Per hop, the smaller of {sources} (occupied acceptors + electrodes) and
-//    {targets} (empty acceptors + electrodes) is walked four members at a time
-//    with a scalar loop while the lanes cover every site of the other role:
-//    row sums R_lane accumulate in registers with no cross-lane traffic.
+//    {targets} (empty acceptors + electrodes) is walked in blocks of four
+//    members (the last block exactly the members left) while the lanes cover
+//    every site of the other role: row sums R_lane accumulate in registers with
+//    no cross-lane traffic.
 //  * Table policies leave the electrodes out of the walked set: with
 //    w = nu * tc[s][e] the rates between acceptor s and all electrodes sum to
 //        sum_e w * min(1, 2^e~_s * 2^-V~_e)   (s occupied, hops into them) or
 //        sum_e w * min(1, 2^-e~_s * 2^V~_e)   (s empty, hops out of them),
-//    one product, min and FMA per electrode and no exp2 per pair
-//    (elec_sums).  The per-trajectory factors 2^+-V~_e sit in LDS.
+//    one product (its clamp is the min) and one FMA per electrode and no exp2
+//    per pair (elec_sums).  The per-trajectory factors 2^+-V~_e sit in LDS.  In
+//    one-wave kernels a hop on which no lane of the other role has its energy
+//    inside the electrodes' voltage window takes per-lane sums fixed for the
+//    trajectory instead.
 //  * Event pick: FP64 wave prefix sum of R (DPP), Philox event uniform * total,
 //    ballot for the first lane whose prefix exceeds it, then one lane-parallel
 //    pass over that lane's pairs (FP32 DPP scan) for the partner site.
