@@ -518,6 +518,7 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
       special = next_special(h, false);
     }
 
+    __builtin_amdgcn_s_setprio(0);
     // roles of the lanes' sites, as wave-uniform lane masks (scalar ops only):
     // sources = occupied acceptors + electrodes, targets = empty acceptors + electrodes
     uint64_t srcm[C], dstm[C];
@@ -576,6 +577,13 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
     for (int c = 0; c < C; ++c)
       R[c] = __builtin_amdgcn_inverse_ballot_w64(udst ? srcm[c] : dstm[c]) ? R[c] : 0.f;
 
+    // From here to the end of the hop the work is one dependency chain (scans,
+    // readlanes, the partner's and the Coulomb columns' LDS loads); the waves
+    // sharing a SIMD fill its gaps with their row sums.  A raised issue
+    // priority lets the chain's instructions go as soon as they are ready
+    // (4096 trajectories: 9 % more hops per second); it drops again for the
+    // next hop's row sums.
+    __builtin_amdgcn_s_setprio(2);
     // FP64 prefix over lane rows
     double incl[C];
     double total = 0.0;
@@ -707,6 +715,7 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
     time += dt;
   }
 
+  __builtin_amdgcn_s_setprio(0);   // a stopped trajectory leaves the loop raised
   if (!team.writer()) return;
   if (lane == 0) {
     const int64_t dt_ticks = wall_clock64() - t_start;
