@@ -518,7 +518,6 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
       special = next_special(h, false);
     }
 
-    __builtin_amdgcn_s_setprio(0);
     // roles of the lanes' sites, as wave-uniform lane masks (scalar ops only):
     // sources = occupied acceptors + electrodes, targets = empty acceptors + electrodes
     uint64_t srcm[C], dstm[C];
@@ -571,18 +570,19 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
     }
     // +1: walked targets, e~_src - e~_dst = v - su; -1: walked sources
     const float sgr = __int_as_float(udst ? (int)0x3f800000 : (int)0xbf800000);
+    __builtin_amdgcn_s_setprio(0);
     row_sums<C>(pol, S, um, team.own(), v, se, sgr, R);
     team.reduce(R, (int)(h & 1), lane);
 #pragma unroll
     for (int c = 0; c < C; ++c)
       R[c] = __builtin_amdgcn_inverse_ballot_w64(udst ? srcm[c] : dstm[c]) ? R[c] : 0.f;
 
-    // From here to the end of the hop the work is one dependency chain (scans,
-    // readlanes, the partner's and the Coulomb columns' LDS loads); the waves
-    // sharing a SIMD fill its gaps with their row sums.  A raised issue
-    // priority lets the chain's instructions go as soon as they are ready
-    // (4096 trajectories: 9 % more hops per second); it drops again for the
-    // next hop's row sums.
+    // From here through the next hop's electrode terms the work is one
+    // dependency chain (scans, readlanes, the partner's and the Coulomb
+    // columns' LDS loads); the waves sharing a SIMD fill its gaps with their
+    // row sums of the walked members.  A raised issue priority lets the
+    // chain's instructions go as soon as they are ready (4096 trajectories:
+    // 9-10 % more hops per second); it drops for the row sums.
     __builtin_amdgcn_s_setprio(2);
     // FP64 prefix over lane rows
     double incl[C];
