@@ -581,9 +581,10 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
     // dependency chain (scans, readlanes, the partner's and the Coulomb
     // columns' LDS loads); the waves sharing a SIMD fill its gaps with their
     // row sums of the walked members.  A raised issue priority lets the
-    // chain's instructions go as soon as they are ready (4096 trajectories:
-    // 9-10 % more hops per second); it drops for the row sums.
-    __builtin_amdgcn_s_setprio(2);
+    // chain's instructions go as soon as they are ready: 3 from the scan to the
+    // partner, 1 for the update and the next hop's masks and electrode terms,
+    // 0 for the row sums (4096 trajectories: 12 % more hops per second).
+    __builtin_amdgcn_s_setprio(3);
     // FP64 prefix over lane rows
     double incl[C];
     double total = 0.0;
@@ -660,6 +661,7 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
       status = ST_ZERO_RATE;
       break;
     }
+    __builtin_amdgcn_s_setprio(1);
     const int src = udst ? sel : pick;
     const int dst = udst ? pick : sel;
     const bool tally = h >= prehops;
