@@ -37,10 +37,10 @@ METRIC = "hops/sec, 4096-traj N=30 ensemble vs plain; p50/p99 latency of served 
 # divides by the LDS read peak measured in the same run (ds_read_b128 stream probe).
 LDS_PEAK_SPEC_GBS = 256 * 256 * 2.4
 # HBM bytes per launch of the batched kernel on the default workload (4096 x 1e5 hops,
-# N=30/P=8, auto table form and workgroup size): rocprofv3 --pmc FETCH_SIZE 390.312 KiB
+# N=30/P=8, auto table form and workgroup size): rocprofv3 --pmc FETCH_SIZE 381.562 KiB
 # and WRITE_SIZE 640 KiB per dispatch, scaled by the copy-kernel calibration factors
-# 2.0 (FETCH) and 1.0 (WRITE): (2 * 390.312 + 640) * 1024 bytes.
-HBM_TRAFFIC_BYTES = 1_454_719
+# 2.0 (FETCH) and 1.0 (WRITE): (2 * 381.562 + 640) * 1024 bytes.
+HBM_TRAFFIC_BYTES = 1_436_799
 # VALU instructions per hop and wave of the batched kernel on the default workload
 # (rocprofv3 --pmc SQ_INSTS_VALU: 4.288e10 per dispatch of 4.096e8 hops), and the VALU
 # issue peak: one wave64 instruction per 4 cycles on each of a CU's 4 SIMDs (16 lanes),
@@ -313,7 +313,7 @@ def main() -> int:
                                 if default_workload else None),
             "note": "algorithmic LDS bytes of the dense formulation / kernel time; peak = "
                     "ds_read_b128 stream measured in this run; the kernel is bound by the VALU "
-                    "its 4 waves per SIMD share (VALUBusy 73%, LdsUtil 9%; 104.7 VALU and 81.7 scalar "
+                    "its 4 waves per SIMD share (VALUBusy 82%, LdsUtil 10%; 104.7 VALU and 81.7 scalar "
                     "instructions per hop): valu_issue_frac = profiled VALU "
                     "instructions per hop x hops / kernel time / (1 per CU and cycle)",
         },
