@@ -204,29 +204,46 @@ def plan_for(trajs: np.ndarray, sizes: dict, has_pos: bool = True, edges=BUCKET_
     for edge, form, w, _ in solo:
         b = edges.index(edge)
         lt = lat.get(edge)
-        # a team entry only where a team is clearly faster alone: a marginal gain
-        # (small networks) does not pay for the launch slot it takes, and a
-        # fourth entry forces the larger sizes' team entries to merge into one
-        # kernel sized for the largest table
-        solo_alone = min(lt["solo"].values()) if lt and lt["solo"] else None
-        if lt and lt["team"] > 1 and solo_alone and lt["lat"] * TEAM_MIN_GAIN <= solo_alone:
-            if team_min_hops is not None:
-                thr = team_min_hops.get(edge)
-            else:
-                # a solo trajectory inside the loaded batch runs slower than alone
-                solo_lat = lt["solo"].get(str(form), lt["lat"]) * BATCH_SLOWDOWN
-                thr = int(math.ceil(t_thr / solo_lat)) if t_thr > 0 else None
-                # only worth a bucket when a trajectory of this size exceeds it
-                sel = bucket == b
-                if thr is not None and not (budget[sel] >= thr).any():
-                    thr = None
-            if thr is not None:
-                teams.append((edge, int(lt["form"]), 1, int(lt["team"]), int(thr)))
+        entry = team_entry(edge, lt, form, t_thr, budget[bucket == b],
+                           None if team_min_hops is None else team_min_hops.get(edge),
+                           fixed=team_min_hops is not None)
+        if entry is not None:
+            teams.append(entry)
         solos.append((edge, form, w, 1, 0))
     N_of = np.array([sizes[int(x)][1] for x in lay])
     max_sn = {e: (int(S_of[bucket == b].max()), int(N_of[bucket == b].max()))
               for b, e in enumerate(edges) if (bucket == b).any()}
     return merge_plan(teams, solos, max_sn, has_pos)
+
+
+def team_entry(edge: int, lt: dict | None, solo_form: int, t_thr: float, budgets: np.ndarray,
+               min_hops: int | None = None, fixed: bool = False):
+    """The team entry of one size bucket, or None.
+
+    lt: the bucket's latency tuning ({"form", "team", "lat", "solo": {form: lat}},
+    seconds per hop of one trajectory alone).  A team entry exists only where a
+    team is at least TEAM_MIN_GAIN times faster alone than the best solo form: a
+    marginal gain (small networks) does not pay for the launch slot it takes,
+    and a fourth entry forces the larger sizes' team entries to merge into one
+    kernel sized for the largest table.  Its threshold takes the trajectories
+    whose solo run (the bucket's throughput form, at BATCH_SLOWDOWN times its
+    lone latency) would outlast the batch's throughput-bound time t_thr, and
+    only when some budget of the bucket reaches it.  fixed: min_hops is the
+    threshold (serving: a property of the request, not of the mix)."""
+    if not lt or lt["team"] <= 1 or not lt["solo"]:
+        return None
+    if lt["lat"] * TEAM_MIN_GAIN > min(lt["solo"].values()):
+        return None
+    if fixed:
+        thr = min_hops
+    else:
+        solo_lat = lt["solo"].get(str(solo_form), lt["lat"]) * BATCH_SLOWDOWN
+        thr = int(math.ceil(t_thr / solo_lat)) if t_thr > 0 else None
+        if thr is not None and not (np.asarray(budgets) >= thr).any():
+            thr = None
+    if thr is None:
+        return None
+    return (edge, int(lt["form"]), 1, int(lt["team"]), int(thr))
 
 
 def merge_plan(teams: list, solos: list, max_sn: dict, has_pos: bool = True,

@@ -99,3 +99,24 @@ def test_merge_plan_fits_the_launch_limit():
     four = autotune.merge_plan(teams[:2], solos[:2], {64: (64, 56), 128: (108, 100)},
                                max_launches=4)
     assert four == teams[:2] + solos[:2]
+
+
+def test_team_entry_rules():
+    """A team entry needs a clear gain alone; its threshold uses the in-batch
+    solo latency and must be reached by some budget of the bucket."""
+    lat = {"form": 2, "team": 4, "lat": 2.0e-6, "solo": {"2": 3.1e-6, "3": 6.0e-6}}
+    budgets = np.array([1e4, 2e5, 8e5])
+    e = autotune.team_entry(192, lat, 3, t_thr=1.5, budgets=budgets)
+    thr = int(np.ceil(1.5 / (6.0e-6 * autotune.BATCH_SLOWDOWN)))
+    assert e == (192, 2, 1, 4, thr)
+    # a marginal team (1.14x at S = 64) takes no launch slot
+    small = {"form": 2, "team": 2, "lat": 0.77e-6, "solo": {"1": 0.88e-6, "2": 0.9e-6}}
+    assert autotune.team_entry(64, small, 1, t_thr=1.5, budgets=np.array([1e6])) is None
+    # no budget reaches the threshold
+    assert autotune.team_entry(192, lat, 3, t_thr=1.5, budgets=np.array([1e4, 5e4])) is None
+    # solo fastest alone
+    assert autotune.team_entry(192, {"form": 2, "team": 1, "lat": 3e-6, "solo": {"2": 3e-6}},
+                               2, t_thr=1.5, budgets=budgets) is None
+    # serving: the threshold is given
+    assert autotune.team_entry(192, lat, 3, 0.0, budgets, min_hops=50_000, fixed=True) == \
+        (192, 2, 1, 4, 50_000)
