@@ -11,7 +11,7 @@ namespace kmc {
 // share its tables in LDS.  Layout: the waves' electrode factors (W*2*PMAX
 // floats), the electrode weights (elw_lds_floats), the pair table
 // (S*S + 64C float2) and, for BT, the c table (N*S + 64C floats).
-template <int C, int W, bool REC, bool BT>
+template <int C, int W, bool REC, bool BT, bool PRIO>
 __global__ void __launch_bounds__(64 * W) kmc_batch_fp32_kernel(BatchParams p) {
   extern __shared__ __align__(16) unsigned char smem[];
   // wave index and everything read from the shared group slot are wave-uniform:
@@ -49,7 +49,7 @@ __global__ void __launch_bounds__(64 * W) kmc_batch_fp32_kernel(BatchParams p) {
     __syncthreads();
     if (wave < gd.count) {
       const TabPolicy<C, BT> pol = TabPolicy<C, BT>::make(tab, ctab, elw, evc, L.S, lane);
-      run_trajectory<C, REC>(p, gd.first + wave, L, pol, lane);
+      run_trajectory<C, REC, TabPolicy<C, BT>, 1, PRIO>(p, gd.first + wave, L, pol, lane);
     }
   }
 }
@@ -77,11 +77,13 @@ __global__ void __launch_bounds__(64 * W) kmc_batch_pos_kernel(BatchParams p) {
   }
 }
 
-#define KMC_INST(C, W)                                                             \
-  template __global__ void kmc_batch_fp32_kernel<C, W, false, false>(BatchParams); \
-  template __global__ void kmc_batch_fp32_kernel<C, W, true, false>(BatchParams);  \
-  template __global__ void kmc_batch_fp32_kernel<C, W, false, true>(BatchParams);  \
-  template __global__ void kmc_batch_fp32_kernel<C, W, true, true>(BatchParams);   \
+#define KMC_INST(C, W)                                                                    \
+  template __global__ void kmc_batch_fp32_kernel<C, W, false, false, true>(BatchParams);  \
+  template __global__ void kmc_batch_fp32_kernel<C, W, true, false, true>(BatchParams);   \
+  template __global__ void kmc_batch_fp32_kernel<C, W, false, true, true>(BatchParams);   \
+  template __global__ void kmc_batch_fp32_kernel<C, W, true, true, true>(BatchParams);    \
+  template __global__ void kmc_batch_fp32_kernel<C, W, false, false, false>(BatchParams); \
+  template __global__ void kmc_batch_fp32_kernel<C, W, false, true, false>(BatchParams);  \
   template __global__ void kmc_batch_pos_kernel<C, W, false>(BatchParams);         \
   template __global__ void kmc_batch_pos_kernel<C, W, true>(BatchParams);
 KMC_INST(1, 1) KMC_INST(1, 2) KMC_INST(1, 4) KMC_INST(1, 8)

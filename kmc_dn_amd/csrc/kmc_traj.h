@@ -394,7 +394,7 @@ __device__ __forceinline__ float pick_lane_value(const float (&x)[C], int site) 
   return r;
 }
 
-template <int C, bool REC, class Pol, int T = 1>
+template <int C, bool REC, class Pol, int T = 1, bool PRIO = true>
 __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, const Pol& pol,
                                int lane, const Team<T, C>& team = Team<T, C>{}) {
   const int64_t t_start = wall_clock64();
@@ -570,7 +570,7 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
     }
     // +1: walked targets, e~_src - e~_dst = v - su; -1: walked sources
     const float sgr = __int_as_float(udst ? (int)0x3f800000 : (int)0xbf800000);
-    __builtin_amdgcn_s_setprio(0);
+    if constexpr (PRIO) __builtin_amdgcn_s_setprio(0);
     row_sums<C>(pol, S, um, team.own(), v, se, sgr, R);
     team.reduce(R, (int)(h & 1), lane);
 #pragma unroll
@@ -583,8 +583,9 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
     // row sums of the walked members.  A raised issue priority lets the
     // chain's instructions go as soon as they are ready: 3 from the scan to the
     // partner, 1 for the update and the next hop's masks and electrode terms,
-    // 0 for the row sums (4096 trajectories: 12 % more hops per second).
-    __builtin_amdgcn_s_setprio(3);
+    // 0 for the row sums (4096 trajectories: 12 % more hops per second).  PRIO
+    // false: the variant for launches whose groups queue behind full SIMDs.
+    if constexpr (PRIO) __builtin_amdgcn_s_setprio(3);
     // FP64 prefix over lane rows
     double incl[C];
     double total = 0.0;
@@ -661,7 +662,7 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
       status = ST_ZERO_RATE;
       break;
     }
-    __builtin_amdgcn_s_setprio(1);
+    if constexpr (PRIO) __builtin_amdgcn_s_setprio(1);
     const int src = udst ? sel : pick;
     const int dst = udst ? pick : sel;
     const bool tally = h >= prehops;
@@ -717,7 +718,7 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
     time += dt;
   }
 
-  __builtin_amdgcn_s_setprio(0);   // a stopped trajectory leaves the loop raised
+  if constexpr (PRIO) __builtin_amdgcn_s_setprio(0);   // a stopped trajectory leaves the loop raised
   if (!team.writer()) return;
   if (lane == 0) {
     const int64_t dt_ticks = wall_clock64() - t_start;
