@@ -37,10 +37,10 @@ METRIC = "hops/sec, 4096-traj N=30 ensemble vs plain; p50/p99 latency of served 
 # divides by the LDS read peak measured in the same run (ds_read_b128 stream probe).
 LDS_PEAK_SPEC_GBS = 256 * 256 * 2.4
 # HBM bytes per launch of the batched kernel on the default workload (4096 x 1e5 hops,
-# N=30/P=8, auto table form and workgroup size): rocprofv3 --pmc FETCH_SIZE 381.562 KiB
+# N=30/P=8, auto table form and workgroup size): rocprofv3 --pmc FETCH_SIZE 384.812 KiB
 # and WRITE_SIZE 640 KiB per dispatch, scaled by the copy-kernel calibration factors
-# 2.0 (FETCH) and 1.0 (WRITE): (2 * 381.562 + 640) * 1024 bytes.
-HBM_TRAFFIC_BYTES = 1_436_799
+# 2.0 (FETCH) and 1.0 (WRITE): (2 * 384.812 + 640) * 1024 bytes.
+HBM_TRAFFIC_BYTES = 1_443_455
 # VALU instructions per hop and wave of the batched kernel on the default workload
 # (rocprofv3 --pmc SQ_INSTS_VALU: 4.288e10 per dispatch of 4.096e8 hops), and the VALU
 # issue peak: one wave64 instruction per 4 cycles on each of a CU's 4 SIMDs (16 lanes),
