@@ -192,24 +192,21 @@ def test_resume_in_chunks_is_bit_exact(gpu_engine):
         assert (r.hops_done == hops).all()
 
 
-def test_team_kernels_deterministic_and_resumable(gpu_engine):
-    """A team of 2 or 4 waves per trajectory: repeated and graph runs agree bit for
+def _team_checks(net, N, P, forms, K, hops):
+    """Teams of 2 or 4 waves per trajectory: repeated and graph runs agree bit for
     bit, a chunked resume run equals the unchunked run, and the currents agree with
-    the solo kernel statistically (the row sums are added in another order)."""
-    net = random_network(4, 70, 10, I_0=50.0, a=0.1, vmax=1.5)
-    N, P = 70, 10
+    the solo positions kernel statistically."""
     lay = engine.Layout(N, P, net["nu"], net["kT"], net["I_0"], net["R"], net["distances"],
                         net["transitions_constant"])
     lay.set_positions(0.1 * net["R"], _positions(net))
     es = lay.energy_sets(net["E_constant"], net["site_energies"][N:])
-    K, hops = 96, 6144
     trajs = engine.make_trajs(K, lay.id, es[0], np.arange(K, dtype=np.uint64), hops,
                               net["occupation"])
     b = engine.Batch(trajs)
     cur = {}
-    for form in (1, 2, 3):
+    for form in forms:
         for team in (1, 2, 4):
-            print(f"[gpu] team={team} form={form}", flush=True)
+            print(f"[gpu] S={N + P} team={team} form={form}", flush=True)
             b.run(precision=_lib.FP32_BATCHED, table_form=form, team=team)
             b.sync()
             r = b.results()
@@ -238,6 +235,19 @@ def test_team_kernels_deterministic_and_resumable(gpu_engine):
         z = np.abs(base.mean(0) - c.mean(0))[ok] / se[ok]
         print(f"[gpu] form/team {key} vs positions solo: max |z| {z.max():.2f}", flush=True)
         assert z.max() < 4.5
+
+
+def test_team_kernels_deterministic_and_resumable(gpu_engine):
+    """Two site chunks (S = 80): the teams add the row sums in another order."""
+    net = random_network(4, 70, 10, I_0=50.0, a=0.1, vmax=1.5)
+    _team_checks(net, 70, 10, (1, 2, 3), K=96, hops=6144)
+
+
+def test_three_chunk_team_kernels(gpu_engine):
+    """Three site chunks (S = 134, whose FP32 table still fits LDS): the team
+    reduction of three row-sum chunks per lane."""
+    net = random_network(14, 122, 12, I_0=50.0, a=0.1, vmax=1.5)
+    _team_checks(net, 122, 12, (2, 3), K=96, hops=4096)
 
 
 def test_budget_buckets_route_long_trajectories_to_teams(gpu_engine):
