@@ -212,7 +212,7 @@ typedef struct kmc_bucket {
   int32_t team;               /* waves per trajectory: 1 (solo), 2 or 4 */
   int64_t min_hops;           /* smallest budget this bucket takes */
   int32_t grid;               /* resident workgroups; 0 = as many as fit */
-  int32_t pad;
+  int32_t s_min;              /* smallest S this bucket takes; 0 = no lower bound */
 } kmc_bucket;
 int kmc_batch_set_buckets(int32_t batch, const kmc_bucket* buckets, int32_t nbuckets);
 

@@ -45,7 +45,7 @@ class kmc_run_opts(C.Structure):
 class kmc_bucket(C.Structure):
     _fields_ = [("s_max", C.c_int32), ("table_form", C.c_int32),
                 ("waves_per_group", C.c_int32), ("team", C.c_int32),
-                ("min_hops", C.c_int64), ("grid", C.c_int32), ("pad", C.c_int32)]
+                ("min_hops", C.c_int64), ("grid", C.c_int32), ("s_min", C.c_int32)]
 
 
 class GoSlice(C.Structure):

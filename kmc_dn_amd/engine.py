@@ -170,13 +170,14 @@ class Batch:
 
     def set_buckets(self, buckets) -> None:
         """Bucket plan: entries (s_max, table_form, waves_per_group[, team[, min_hops[,
-        grid]]]); a trajectory goes to the first entry with S <= s_max and budget >=
-        min_hops.  Each bucket runs in its own kernel, all in one fork/join.  [] clears."""
+        grid[, s_min]]]]); a trajectory goes to the first entry with s_min <= S <=
+        s_max and budget >= min_hops.  Each bucket runs in its own kernel, all in one
+        fork/join.  [] clears."""
         arr = (L.kmc_bucket * max(1, len(buckets)))()
         for i, b in enumerate(buckets):
-            b = [int(x) for x in b] + [0] * (6 - len(b))
-            s_max, form, w, team, min_hops, grid = b[:6]
-            arr[i] = L.kmc_bucket(s_max, form, w, max(1, team), min_hops, grid, 0)
+            b = [int(x) for x in b] + [0] * (7 - len(b))
+            s_max, form, w, team, min_hops, grid, s_min = b[:7]
+            arr[i] = L.kmc_bucket(s_max, form, w, max(1, team), min_hops, grid, s_min)
         L.check(L.load().kmc_batch_set_buckets(self.id, arr, len(buckets)))
 
     def update(self, trajs: np.ndarray) -> None:

@@ -250,6 +250,43 @@ def test_three_chunk_team_kernels(gpu_engine):
     _team_checks(net, 122, 12, (2, 3), K=96, hops=4096)
 
 
+def test_bucket_size_lower_bound(gpu_engine):
+    """s_min bounds a bucket from below: the larger layout's trajectories take the
+    team entry in front, the smaller one's fall through to the solo entry, each
+    bit for bit as in a batch of its own; s_min > s_max is refused."""
+    lays, parts = [], []
+    for seed, N, P in ((21, 40, 8), (22, 80, 8)):   # S = 48 and 88
+        net = random_network(seed, N, P, I_0=50.0, a=0.1, vmax=1.0)
+        lay = engine.Layout(N, P, net["nu"], net["kT"], net["I_0"], net["R"], net["distances"],
+                            net["transitions_constant"])
+        es = lay.energy_sets(net["E_constant"], net["site_energies"][N:])
+        lays.append(lay)
+        parts.append(engine.make_trajs(32, lay.id, es[0], np.arange(32, dtype=np.uint64) + 100 * seed,
+                                       3000, net["occupation"]))
+    b = engine.Batch(np.concatenate(parts))
+    print("[gpu] s_min: S >= 60 -> team 4, the rest solo", flush=True)
+    b.set_buckets([(128, 2, 1, 4, 0, 0, 60), (128, 2, 4, 1, 0)])
+    b.run(precision=_lib.FP32_BATCHED, use_graph=True)
+    b.sync()
+    mixed = b.results()
+    with pytest.raises(_lib.KmcError) as e:
+        b.set_buckets([(64, 2, 1, 4, 0, 0, 65)])
+    assert e.value.code == _lib.ERR_ARGS
+    b.close()
+    for i, team in ((0, 1), (1, 4)):
+        one = engine.Batch(parts[i])
+        one.run(precision=_lib.FP32_BATCHED, table_form=2, team=team)
+        one.sync()
+        r = one.results()
+        one.close()
+        sel = slice(32 * i, 32 * i + 32)
+        assert (r.status == 0).all()
+        assert np.array_equal(r.eo, mixed.eo[sel]) and np.array_equal(r.time, mixed.time[sel])
+        assert np.array_equal(r.occ_bits, mixed.occ_bits[sel])
+    for lay in lays:
+        lay.close()
+
+
 def test_budget_buckets_route_long_trajectories_to_teams(gpu_engine):
     net = random_network(6, 40, 8, I_0=50.0, a=0.1, vmax=1.0)
     N, P = 40, 8
