@@ -20,7 +20,7 @@ def kernel_stats(con) -> list[tuple]:
 
 def dispatches(con) -> list[tuple]:
     return con.execute("select name, grid_x, workgroup_x, lds_size, vgpr_count, "
-                       "scratch_size, duration from kernels order by start").fetchall()
+                       "scratch_size, duration, start from kernels order by start").fetchall()
 
 
 def counter_stats(con) -> dict:
@@ -51,11 +51,15 @@ def main(argv=None) -> int:
             for name, calls, tot, avg, pct in ks:
                 print(f"| `{name}` | {calls} | {tot / 1e3:.3f} | {avg / 1e3:.3f} | {pct:.2f} |")
         if args.dispatches:
-            print("| kernel | grid | workgroup | LDS B | VGPRs | scratch | ms |")
-            print("|---|---|---|---|---|---|---|")
-            for name, gx, wx, lds, vg, scr, dur in dispatches(con):
+            # start: ms after the first listed dispatch (concurrent launches overlap)
+            print("| kernel | grid | workgroup | LDS B | VGPRs | scratch | start ms | ms |")
+            print("|---|---|---|---|---|---|---|---|")
+            t0 = None
+            for name, gx, wx, lds, vg, scr, dur, start in dispatches(con):
                 if args.match in name:
-                    print(f"| `{name}` | {gx} | {wx} | {lds} | {vg} | {scr} | {dur / 1e6:.3f} |")
+                    t0 = start if t0 is None else t0
+                    print(f"| `{name}` | {gx} | {wx} | {lds} | {vg} | {scr} | "
+                          f"{(start - t0) / 1e6:.3f} | {dur / 1e6:.3f} |")
         cs = counter_stats(con)
         if cs:
             print("| kernel | counter | dispatches | average per dispatch |")
