@@ -179,7 +179,9 @@ def plan_for(trajs: np.ndarray, sizes: dict, has_pos: bool = True, edges=BUCKET_
              log=None) -> list:
     """Launch plan for a batch: per size bucket a solo bucket tuned for throughput
     and, in front of it, a team bucket tuned for hop latency that takes the
-    trajectories whose solo run would outlast the batch's throughput-bound time.
+    trajectories whose solo run would outlast the batch's throughput-bound time;
+    merged to the launch limit, then two team entries re-cut by LDS
+    (split_team_entries).
 
     team_min_hops: {edge: budget} to fix the team threshold per bucket instead
     (serving: a property of the request, not of the current mix)."""
@@ -213,7 +215,43 @@ def plan_for(trajs: np.ndarray, sizes: dict, has_pos: bool = True, edges=BUCKET_
     N_of = np.array([sizes[int(x)][1] for x in lay])
     max_sn = {e: (int(S_of[bucket == b].max()), int(N_of[bucket == b].max()))
               for b, e in enumerate(edges) if (bucket == b).any()}
-    return merge_plan(teams, solos, max_sn, has_pos)
+    return split_team_entries(merge_plan(teams, solos, max_sn, has_pos),
+                              int(S_of.max()), int(N_of.max()))
+
+
+def team_lds_bytes(form: int, S: int, N: int, team: int) -> int:
+    """Upper bound of the LDS bytes of a table-form team workgroup: the table
+    terms plus the team's partial row sums (kmc_team.hip team_lds_bytes)."""
+    C = (S + 63) // 64
+    return table_lds_bytes(form, S, N, team) + 2 * team * C * 64 * 4
+
+
+def split_team_entries(plan: list, s_max: int, n_max: int) -> list:
+    """Two table-form team entries of one form and team size become one entry
+    for the sizes whose team workgroup fits twice into a CU's LDS (S <= cut) and
+    one, bounded below by s_min = cut + 1, for the larger sizes, both at the
+    lower of the two thresholds.  A team entry whose table takes most of the
+    LDS runs one trajectory per CU, so its throughput, not the hop latency of
+    its longest trajectory, bounded the 2048-network uneven batch (2.07 s with
+    entries cut at S = 128, 1.92-1.94 s cut at S = 90).  The larger entry keeps
+    a kernel of as many site chunks as its largest network needs; its table
+    must fit the LDS.  Other plans are returned unchanged."""
+    teams = [e for e in plan if e[3] > 1]
+    if len(teams) != 2 or any(len(e) > 5 for e in teams):
+        return plan
+    form, team = teams[0][1], teams[0][3]
+    if form == 3 or teams[1][1] != form or teams[1][3] != team:
+        return plan
+    cut = 0
+    for S in range(1, s_max):
+        if team_lds_bytes(form, S, min(n_max, S), team) <= LDS_BYTES // 2:
+            cut = S
+    if cut == 0 or team_lds_bytes(form, s_max, n_max, team) > LDS_BYTES:
+        return plan
+    thr = min(teams[0][4], teams[1][4])
+    top = max(teams[0][0], teams[1][0])
+    return [(top, form, 1, team, thr, 0, cut + 1), (cut, form, 1, team, thr)] + \
+        [e for e in plan if e[3] <= 1]
 
 
 def team_entry(edge: int, lt: dict | None, solo_form: int, t_thr: float, budgets: np.ndarray,

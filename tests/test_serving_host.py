@@ -101,6 +101,30 @@ def test_merge_plan_fits_the_launch_limit():
     assert four == teams[:2] + solos[:2]
 
 
+def test_split_team_entries():
+    """Two table-form team entries are cut where a team workgroup fits twice into
+    a CU's LDS; the larger sizes take an entry bounded below by s_min."""
+    plan = [(128, 2, 1, 4, 212_420), (192, 2, 1, 4, 256_712), (192, 3, 4, 1, 0)]
+    out = autotune.split_team_entries(plan, 132, 120)
+    assert out == [(192, 2, 1, 4, 212_420, 0, 93), (92, 2, 1, 4, 212_420), (192, 3, 4, 1, 0)]
+    assert autotune.team_lds_bytes(2, 92, 92, 4) <= autotune.LDS_BYTES // 2
+    assert autotune.team_lds_bytes(2, 93, 93, 4) > autotune.LDS_BYTES // 2
+    assert autotune.team_lds_bytes(2, 132, 120, 4) <= autotune.LDS_BYTES
+    # first match with s_min: every (S, budget) has one entry; long ones in teams
+    for S in (15, 64, 92, 93, 128, 132):
+        for budget in (10_000, 212_420, 1_000_000):
+            hit = [e for e in out if (e + (0, 0))[6] <= S <= e[0] and budget >= e[4]][0]
+            assert (hit[3] > 1) == (budget >= 212_420)
+            assert hit[3] == 1 or (hit[0] == 92) == (S <= 92)
+    # unchanged: one team entry, mixed forms, positions teams, a table too large
+    assert autotune.split_team_entries(plan[1:], 132, 120) == plan[1:]
+    mixed = [(64, 1, 1, 4, 900_000), (192, 2, 1, 4, 270_000), (192, 3, 4, 1, 0)]
+    assert autotune.split_team_entries(mixed, 132, 120) == mixed
+    pos = [(128, 3, 1, 4, 1000), (192, 3, 1, 4, 2000), (192, 3, 4, 1, 0)]
+    assert autotune.split_team_entries(pos, 132, 120) == pos
+    assert autotune.split_team_entries(plan, 150, 138) == plan
+
+
 def test_team_entry_rules():
     """A team entry needs a clear gain alone; its threshold uses the in-batch
     solo latency and must be reached by some budget of the bucket."""
