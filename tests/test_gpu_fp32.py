@@ -287,6 +287,39 @@ def test_bucket_size_lower_bound(gpu_engine):
         lay.close()
 
 
+def test_team_kernel_chunk_count_is_irrelevant(gpu_engine):
+    """An S = 100 layout's trajectories in a team entry whose largest network has
+    S = 130 (a kernel of three site chunks) equal, bit for bit, the same
+    trajectories in a batch of their own (two site chunks): the third chunk's
+    guards add nothing (the team entries of a plan cut by LDS run so)."""
+    lays, parts = [], []
+    for seed, N, P in ((31, 90, 10), (32, 118, 12)):   # S = 100 and 130
+        net = random_network(seed, N, P, I_0=50.0, a=0.1, vmax=1.0)
+        lay = engine.Layout(N, P, net["nu"], net["kT"], net["I_0"], net["R"], net["distances"],
+                            net["transitions_constant"])
+        es = lay.energy_sets(net["E_constant"], net["site_energies"][N:])
+        lays.append(lay)
+        parts.append(engine.make_trajs(16, lay.id, es[0], np.arange(16, dtype=np.uint64) + 100 * seed,
+                                       3000, net["occupation"]))
+    print("[gpu] S=100 trajectories in a three-chunk team kernel", flush=True)
+    b = engine.Batch(np.concatenate(parts))
+    b.set_buckets([(192, 2, 1, 4, 0)])
+    b.run(precision=_lib.FP32_BATCHED, use_graph=True)
+    b.sync()
+    mixed = b.results()
+    b.close()
+    one = engine.Batch(parts[0])
+    one.run(precision=_lib.FP32_BATCHED, table_form=2, team=4)
+    one.sync()
+    r = one.results()
+    one.close()
+    assert (r.status == 0).all() and (mixed.status == 0).all()
+    assert np.array_equal(r.eo, mixed.eo[:16]) and np.array_equal(r.time, mixed.time[:16])
+    assert np.array_equal(r.occ_bits, mixed.occ_bits[:16])
+    for lay in lays:
+        lay.close()
+
+
 def test_budget_buckets_route_long_trajectories_to_teams(gpu_engine):
     net = random_network(6, 40, 8, I_0=50.0, a=0.1, vmax=1.0)
     N, P = 40, 8
