@@ -228,8 +228,9 @@ int kmc_batch_results(int32_t batch, double* time, int64_t* eo, uint64_t* occ, i
 /* Device time of each trajectory in the last FP32 run (caller order, seconds,
  * from the GPU's constant-rate counter): what a scheduler sizes hop ranges by. */
 int kmc_batch_timing(int32_t batch, double* seconds);
-/* Net traffic of the last FP32 run with record bit 1: traffic[ntraj*stride],
- * trajectory j's S*S matrix at j*stride (stride returned in *stride_out). */
+/* Net traffic of the last run when it was an FP32 run with record bit 1 (fails
+ * otherwise): traffic[ntraj*stride], trajectory j's S*S matrix at j*stride
+ * (stride returned in *stride_out).  Batched FP64 runs refuse record bit 1. */
 int kmc_batch_traffic(int32_t batch, double* traffic, int64_t* stride_out);
 int kmc_batch_destroy(int32_t batch);
 
@@ -273,13 +274,22 @@ double wrapperSimulateRecordPlus(int64_t NSites, int64_t NElectrodes, double nu,
                                  kmc_GoSlice electrode_occupation, kmc_GoSlice site_energies,
                                  int64_t hops, uint8_t record, kmc_GoSlice traffic,
                                  kmc_GoSlice average_occupation);
-/* simulationWrapper.go:112-125, :218-233, :171-216 -- not hop loops: return NaN */
+/* simulationWrapper.go:112-125: the FP32 hop loop from an empty network, then
+ * calcAverageCurrent (simulation.go:671-736) at its time-averaged occupation;
+ * electrode_occupation = those currents x time (same argument list as
+ * wrapperSimulate). */
 double wrapperSimulateCombined(int64_t, int64_t, double, double, double, double, kmc_GoSlice,
                                kmc_GoSlice, kmc_GoSlice, kmc_GoSlice, kmc_GoSlice, kmc_GoSlice,
                                int64_t, uint8_t, kmc_GoSlice, kmc_GoSlice);
+/* simulationWrapper.go:218-233: every acceptor at occupation 0.5, then
+ * probSimulate (probabilitySimulation.go:53-157) for `hops` steps through
+ * kmc_meanfield_simulate; occupation, site_energies[0:N] and
+ * electrode_occupation are written back. */
 double wrapperSimulateProbability(int64_t, int64_t, double, double, double, double, kmc_GoSlice,
                                   kmc_GoSlice, kmc_GoSlice, kmc_GoSlice, kmc_GoSlice, kmc_GoSlice,
                                   int64_t, uint8_t, kmc_GoSlice, kmc_GoSlice);
+/* simulationWrapper.go:171-216, a statistic of the Go occupation cache, which
+ * this engine does not keep: returns NaN. */
 double analyzeStateOverlap(int64_t, int64_t, double, double, double, double, kmc_GoSlice,
                            kmc_GoSlice, kmc_GoSlice, kmc_GoSlice, kmc_GoSlice, kmc_GoSlice, int64_t,
                            uint8_t, kmc_GoSlice, kmc_GoSlice);

@@ -94,6 +94,14 @@ KMC_INST(3, 1) KMC_INST(3, 2) KMC_INST(3, 4) KMC_INST(3, 8)
 // Zeroes the work-queue head (a kernel node, so a captured graph can replay it).
 __global__ void kmc_queue_reset_kernel(int32_t* q) { *q = 0; }
 
+// Zeroes n doubles (grid-stride): clears record buffers on the stream, also
+// inside a captured graph, where a memset node is not used.
+__global__ void kmc_fill_zero_kernel(double* p, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = 0.0;
+}
+
 // Scatters n staged energy sets into their device slots, one workgroup per set:
 // set i (id ids[i], nps[2i] acceptors, nps[2i+1] electrodes) takes E[i*NMAX ...]
 // and V[i*PMAX ...].

@@ -27,6 +27,7 @@ __global__ void kmc_batch_fp32_kernel(BatchParams);
 template <int C, int W, bool REC>
 __global__ void kmc_batch_pos_kernel(BatchParams);
 __global__ void kmc_queue_reset_kernel(int32_t*);
+__global__ void kmc_fill_zero_kernel(double*, int64_t);
 __global__ void kmc_eset_scatter_kernel(const EsetDev*, const int32_t*, const int32_t*, const double*,
                                         const double*, int64_t);
 size_t plain_lds_bytes(int S, int N, bool stage);
@@ -114,12 +115,14 @@ struct Batch {
   int smax = 0, nmax = 0, pmax = 0;
   DevBuf trajs, out, occ_time, traffic, queue, groups;
   int32_t ngroups = 0;
-  bool last_rec = false;
-  bool has_occ_time = false;
-  bool has_traffic = false;
+  bool last_rec = false;       // the last run recorded occupation time (record bit 0)
+  bool last_traffic = false;   // the last run was an FP32 run with traffic (record bit 1)
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   hipGraphExec_t graph = nullptr;
-  int64_t graph_key = -1;
+  // what the captured graph was built from: the run options and the registry
+  // generation (the device addresses of the layout and energy-set arrays)
+  kmc_run_opts graph_opts{};
+  uint64_t graph_gen = 0;
   // size-bucket plan: per bucket its groups, largest S / N, and a side stream
   // with fork/join events so that the bucket kernels run side by side
   std::vector<kmc_bucket> plan;
@@ -159,6 +162,9 @@ struct Engine {
   DevBuf stage;   // staged energy sets of kmc_eset_write_many
   std::vector<Batch*> batches;
   bool layouts_dirty = true, esets_dirty = true;
+  // bumped whenever layouts_dev or esets_dev moves to a new device address:
+  // a captured graph holds those addresses as kernel arguments
+  uint64_t reg_gen = 1;
 };
 
 Engine& eng() {
@@ -185,17 +191,21 @@ int ensure_init() {
 
 int sync_registry() {
   Engine& E = eng();
+  const void* lp = E.layouts_dev.p;
+  const void* ep = E.esets_dev.p;
   if (E.layouts_dirty) {
     std::vector<LayoutDev> v;
     v.reserve(E.layouts.size());
     for (Layout* L : E.layouts) v.push_back(L ? L->dev : LayoutDev{});
     int st = E.layouts_dev.alloc(std::max<size_t>(1, v.size()) * sizeof(LayoutDev));
+    if (E.layouts_dev.p != lp) ++E.reg_gen;
     if (st) return st;
     if (!v.empty()) HIPOK(hipMemcpy(E.layouts_dev.p, v.data(), v.size() * sizeof(LayoutDev), hipMemcpyHostToDevice));
     E.layouts_dirty = false;
   }
   if (E.esets_dirty) {
     int st = E.esets_dev.alloc(std::max<size_t>(1, E.esets_host.size()) * sizeof(EsetDev));
+    if (E.esets_dev.p != ep) ++E.reg_gen;
     if (st) return st;
     if (!E.esets_host.empty())
       HIPOK(hipMemcpy(E.esets_dev.p, E.esets_host.data(), E.esets_host.size() * sizeof(EsetDev),
@@ -604,19 +614,41 @@ int batch_create_locked(const kmc_traj* t, int64_t ntraj, int32_t* out) {
   return KMC_OK;
 }
 
+// The record buffers (occupation time, traffic) are sized when a run needs
+// them (ensure_occ_time / ensure_traffic), against the batch's current
+// capacity and largest S, so an update to more trajectories or larger
+// networks never leaves them short.
 int batch_update_locked(Batch* B, const kmc_traj* t, int64_t ntraj) {
   (void)hipStreamSynchronize(eng().stream);
   int st = batch_prepare(B, t, ntraj);
   if (st) return st;
+  B->last_rec = B->last_traffic = false;
   if (ntraj > B->capacity) {
     if ((st = B->out.alloc(sizeof(TrajOut) * ntraj))) return st;
-    if (B->has_occ_time && (st = B->occ_time.alloc(sizeof(double) * NMAX * ntraj))) return st;
-    B->has_traffic = false;
     B->capacity = ntraj;
     return upload(B->trajs, B->host);
   }
   HIPOK(hipMemcpy(B->trajs.p, B->host.data(), sizeof(TrajDev) * ntraj, hipMemcpyHostToDevice));
   return KMC_OK;
+}
+
+// Occupation time of every trajectory slot (NMAX doubles each), zeroed when
+// (re)allocated; kernels overwrite each recorded slot at the end of a run.
+int ensure_occ_time(Batch* B) {
+  const size_t need = sizeof(double) * NMAX * (size_t)B->capacity;
+  if (B->occ_time.p && B->occ_time.cap >= need) return KMC_OK;
+  int st = B->occ_time.alloc(need);
+  if (st) return st;
+  HIPOK(hipMemset(B->occ_time.p, 0, need));
+  return KMC_OK;
+}
+
+// Traffic matrices of every trajectory slot (smax^2 doubles each); zeroed on
+// the stream by a fill kernel before each non-resume run.
+int ensure_traffic(Batch* B) {
+  const size_t need = sizeof(double) * (size_t)B->smax * B->smax * (size_t)B->capacity;
+  if (B->traffic.p && B->traffic.cap >= need) return KMC_OK;
+  return B->traffic.alloc(need);
 }
 
 int batch_set_state_locked(Batch* B, const int64_t* hb, const int64_t* he, const double* time,
@@ -804,20 +836,9 @@ int setup_launch(Batch* B, const kmc_run_opts& o, int form, int W, int team, int
 
 // Batch-wide record buffers (occupation time, traffic).
 int prepare_record(Batch* B, const kmc_run_opts& o) {
-  if ((o.record & 1) && !B->has_occ_time) {
-    int st = B->occ_time.alloc(sizeof(double) * NMAX * B->capacity);
-    if (st) return st;
-    HIPOK(hipMemset(B->occ_time.p, 0, sizeof(double) * NMAX * B->capacity));
-    B->has_occ_time = true;
-  }
-  if (o.record & 2) {
-    if (o.use_graph) return fail(KMC_ERR_ARGS, "traffic recording is not available with use_graph");
-    if (!B->has_traffic) {
-      int st = B->traffic.alloc(sizeof(double) * (size_t)B->smax * B->smax * B->capacity);
-      if (st) return st;
-      B->has_traffic = true;
-    }
-  }
+  int st;
+  if ((o.record & 1) && (st = ensure_occ_time(B))) return st;
+  if ((o.record & 2) && (st = ensure_traffic(B))) return st;
   return KMC_OK;
 }
 
@@ -945,8 +966,15 @@ int ensure_side_streams(Batch* B, int n) {
 // Launches on stream s; several launches fork onto the batch's side streams
 // and join back into s (parallel branches when captured into a graph).
 int launch_fp32(Batch* B, const kmc_run_opts& o, const std::vector<Fp32Launch>& Ls, hipStream_t s) {
-  if ((o.record & 2) && !o.resume)
-    HIPOK(hipMemsetAsync(B->traffic.p, 0, sizeof(double) * (size_t)B->smax * B->smax * B->ntraj, s));
+  if ((o.record & 2) && !o.resume) {
+    // a kernel node, not a memset node: this may be recorded into a graph
+    double* tp = B->traffic.as<double>();
+    int64_t n = (int64_t)B->smax * B->smax * B->ntraj;
+    void* za[] = {&tp, &n};
+    const unsigned blocks = (unsigned)std::min<int64_t>(2048, (n + 255) / 256);
+    HIPOK(hipLaunchKernel((const void*)kmc_fill_zero_kernel, dim3(std::max(1u, blocks)), dim3(256),
+                          za, 0, s));
+  }
   if (Ls.size() == 1) return launch_one(Ls[0], s);
   // launch 0 stays on s, the others fork onto side streams: n launches use n
   // streams (HIP maps streams onto a few hardware queues; launches whose
@@ -981,10 +1009,9 @@ int enqueue_fp64(Batch* B, const kmc_run_opts& o, hipStream_t s, const double* u
   if (B->smax > SMAX_FP64)
     return fail(KMC_ERR_LIMIT, "FP64 dense kernel supports S <= %d (got %d)", SMAX_FP64, B->smax);
   const bool rec = o.record != 0 || traffic != nullptr;
-  if (rec && !B->has_occ_time) {
-    int st = B->occ_time.alloc(sizeof(double) * NMAX * B->ntraj);
+  if (rec) {
+    int st = ensure_occ_time(B);
     if (st) return st;
-    B->has_occ_time = true;
   }
   const size_t cap = (size_t)160 * 1024;
   const bool stage = plain_lds_bytes(B->smax, B->nmax, true) <= cap;
@@ -1020,11 +1047,15 @@ int batch_run_locked(int32_t id, const kmc_run_opts* opts) {
   hipStream_t s = E.stream;
   if (o.precision == KMC_FP64_DENSE) {
     if (o.resume) return fail(KMC_ERR_UNSUPPORTED, "resume runs use the FP32 batched kernel");
+    if (o.record & 2)
+      return fail(KMC_ERR_UNSUPPORTED, "batched FP64 runs record no traffic (record bit 1); "
+                                       "kmc_simulate_discrete records the reference's gross traffic");
     HIPOK(hipEventRecord(B->ev0, s));
     st = enqueue_fp64(B, o, s, nullptr, nullptr, nullptr, nullptr, nullptr);
     if (st) return st;
     HIPOK(hipEventRecord(B->ev1, s));
     B->last_rec = (o.record & 1) != 0;
+    B->last_traffic = false;
     return KMC_OK;
   }
   if (o.precision != KMC_FP32_BATCHED) return fail(KMC_ERR_ARGS, "unknown precision %d", o.precision);
@@ -1032,12 +1063,11 @@ int batch_run_locked(int32_t id, const kmc_run_opts* opts) {
   if ((st = prepare_fp32(B, o, Ls))) return st;
   if (Ls.size() > 1 && (st = ensure_side_streams(B, (int)Ls.size()))) return st;
   if (o.use_graph) {
-    const int64_t key = (int64_t)(o.waves_per_group & 0xFF) | ((int64_t)(o.record & 1) << 8) |
-                        ((int64_t)(o.post_hop_record & 1) << 9) |
-                        ((int64_t)(o.recompute_interval & 0xFFFFF) << 10) |
-                        ((int64_t)(o.table_form & 3) << 30) | ((int64_t)(o.resume & 1) << 32) |
-                        ((int64_t)(o.team & 7) << 33);
-    if (!B->graph || B->graph_key != key || o.grid_limit) {
+    // a graph replays its kernels' arguments as captured: re-capture when the
+    // options or the registry's device addresses differ from that capture
+    const bool same = B->graph && B->graph_gen == E.reg_gen &&
+                      memcmp(&B->graph_opts, &o, sizeof o) == 0;
+    if (!same) {
       if (B->graph) {
         (void)hipGraphExecDestroy(B->graph);
         B->graph = nullptr;
@@ -1055,7 +1085,8 @@ int batch_run_locked(int32_t id, const kmc_run_opts* opts) {
       hipError_t ie = hipGraphInstantiate(&B->graph, g, nullptr, nullptr, 0);
       (void)hipGraphDestroy(g);
       if (ie != hipSuccess) return fail(KMC_ERR_DEVICE, "graph instantiate: %s", hipGetErrorString(ie));
-      B->graph_key = key;
+      B->graph_opts = o;
+      B->graph_gen = E.reg_gen;
     }
     HIPOK(hipEventRecord(B->ev0, s));
     HIPOK(hipGraphLaunch(B->graph, s));
@@ -1067,6 +1098,7 @@ int batch_run_locked(int32_t id, const kmc_run_opts* opts) {
     HIPOK(hipEventRecord(B->ev1, s));
   }
   B->last_rec = (o.record & 1) != 0;
+  B->last_traffic = (o.record & 2) != 0;
   return KMC_OK;
 }
 
@@ -1267,7 +1299,8 @@ int kmc_batch_results(int32_t batch, double* time, int64_t* eo, uint64_t* occ, i
   HIPOK(hipMemcpy(out.data(), B->out.p, sizeof(TrajOut) * B->ntraj, hipMemcpyDeviceToHost));
   std::vector<double> ot;
   if (occ_time) {
-    if (!B->has_occ_time || !B->last_rec) return fail(KMC_ERR_ARGS, "batch was not run with record=1");
+    if (!B->last_rec || B->occ_time.cap < sizeof(double) * NMAX * (size_t)B->ntraj)
+      return fail(KMC_ERR_ARGS, "the last run of this batch did not record occupation time (record bit 0)");
     ot.resize((size_t)NMAX * B->ntraj);
     HIPOK(hipMemcpy(ot.data(), B->occ_time.p, sizeof(double) * ot.size(), hipMemcpyDeviceToHost));
   }
@@ -1308,7 +1341,8 @@ int kmc_batch_traffic(int32_t batch, double* traffic, int64_t* stride_out) {
   std::lock_guard<std::mutex> lk(E.mu);
   Batch* B = get_batch(batch);
   if (!B) return fail(KMC_ERR_ARGS, "bad batch id %d", batch);
-  if (!B->has_traffic) return fail(KMC_ERR_ARGS, "batch was not run with record bit 1 (traffic)");
+  if (!B->last_traffic)
+    return fail(KMC_ERR_ARGS, "the last run of this batch recorded no traffic (FP32 run with record bit 1)");
   const int64_t stride = (int64_t)B->smax * B->smax;
   if (stride_out) *stride_out = stride;
   if (!traffic) return KMC_OK;

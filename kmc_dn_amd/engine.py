@@ -227,13 +227,19 @@ class Batch:
         L.check(L.load().kmc_batch_timing(self.id, L.ptr(sec, L._dp)))
         return sec
 
-    def traffic(self, sizes) -> list:
-        """Net traffic of the last FP32 run with traffic=True: one S_j x S_j matrix per
-        trajectory j, ``sizes[j]`` = S_j of its layout."""
+    def traffic_flat(self) -> np.ndarray:
+        """Net traffic of the last run (an FP32 run with traffic=True), (n, stride):
+        row j holds trajectory j's S_j x S_j matrix row-major from its start."""
         stride = C.c_int64()
         L.check(L.load().kmc_batch_traffic(self.id, None, C.byref(stride)))
         flat = np.zeros((self.n, stride.value))
         L.check(L.load().kmc_batch_traffic(self.id, L.ptr(flat, L._dp), C.byref(stride)))
+        return flat
+
+    def traffic(self, sizes) -> list:
+        """Net traffic of the last FP32 run with traffic=True: one S_j x S_j matrix per
+        trajectory j, ``sizes[j]`` = S_j of its layout."""
+        flat = self.traffic_flat()
         sizes = np.broadcast_to(np.asarray(sizes), (self.n,))
         return [flat[j, :S * S].reshape(S, S) for j, S in enumerate(sizes)]
 

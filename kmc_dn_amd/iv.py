@@ -41,13 +41,20 @@ def currents_from(res: engine.BatchResult, nV: int, replicas: int, P: int):
 
 
 def IV(kmc, electrode: int, voltagelist, hops: int = 0, prehops: int = 0, replicas: int = 1,
-       precision: int = _lib.FP32_BATCHED, seed0: int | None = None, return_stderr=False, **_):
+       precision: int = _lib.FP32_BATCHED, seed0: int | None = None, return_stderr=False,
+       start: str = "empty", **_):
     """currentlist[P, n] of a kmc_dn model with `electrode` swept over `voltagelist`.
 
-    Every point starts from the model's current occupation and runs `prehops`
-    untallied hops before `hops` tallied ones (the reference's Go path restarts
-    each point from an empty network, simulationWrapper.go:134-141).
+    start="empty" (default) is the reference's behaviour: each point runs
+    go_simulation(goSpecificFunction='wrapperSimulateRecord') (utils :683-684),
+    whose Go wrapper starts from an empty network and tallies every hop
+    (simulationWrapper.go:134-141); the reference's numba prehops move only the
+    model's own occupation, which that run ignores, so `prehops` adds nothing
+    here.  start="model" instead starts every point from the model's current
+    occupation and runs `prehops` untallied hops before the `hops` tallied ones.
     """
+    if start not in ("empty", "model"):
+        raise ValueError("start must be 'empty' or 'model'")
     if hops <= 0:
         raise _lib.KmcError(_lib.ERR_UNSUPPORTED, "convergence-based IV (hops=0) is not supported")
     volts = np.asarray(voltagelist, dtype=np.float64)
@@ -58,7 +65,11 @@ def IV(kmc, electrode: int, voltagelist, hops: int = 0, prehops: int = 0, replic
     E = basis.eV(Vs, kmc.mu) + kmc.comp_constant[None, :]
     lay = kmc._layout()
     seed = int(np.random.randint(0, 2 ** 31)) if seed0 is None else seed0
-    b, ids = iv_batch(lay, E, Vs, replicas, int(hops), kmc.occupation, int(prehops), seed)
+    if start == "empty":
+        occ0, pre = np.zeros(kmc.N, dtype=bool), 0
+    else:
+        occ0, pre = kmc.occupation, int(prehops)
+    b, ids = iv_batch(lay, E, Vs, replicas, int(hops), occ0, pre, seed)
     b.run(precision=precision)
     b.sync()
     r = b.results()

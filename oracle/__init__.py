@@ -64,7 +64,7 @@ def lib():
         L.oracle_go_simulate.argtypes = [
             C.c_int64, C.c_int64, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
             _dp, _dp, _dp, _dp, _u8p, _dp, C.c_int64, C.c_int, C.c_int, C.c_int, _dp, _dp,
-            _dp, C.c_uint64, C.c_uint64, C.c_int64, _i32p, _dp]
+            _dp, C.c_uint64, C.c_uint64, C.c_int64, _i32p, _dp, _i64p]
         L.oracle_go_simulate.restype = C.c_int
         L.oracle_numba_ensemble.argtypes = [
             C.c_int64, C.c_int64, C.c_double, C.c_double, C.c_double, C.c_double,
@@ -74,7 +74,7 @@ def lib():
         L.oracle_go_ensemble.argtypes = [
             C.c_int64, C.c_int64, C.c_double, C.c_double, C.c_double, C.c_double,
             _dp, _dp, _dp, _dp, _u8p, C.c_int64, _u64p, C.c_uint64, C.c_int64, C.c_int, C.c_int,
-            C.c_int64, C.c_int, _dp, _dp, _u8p, _i32p]
+            C.c_int64, C.c_int, C.c_double, _dp, _dp, _u8p, _i32p, _i64p]
         L.oracle_go_ensemble.restype = C.c_int
         L.oracle_prob_simulate.argtypes = [
             C.c_int64, C.c_int64, C.c_double, C.c_double, C.c_double, C.c_double,
@@ -146,7 +146,7 @@ def numba_simulate(N, P, nu, kT, I_0, R, distances, E_constant, site_energies,
 
 class GoResult:
     __slots__ = ("time", "occupation", "electrode_occupation", "site_energies", "traffic",
-                 "average_occupation", "events", "status")
+                 "average_occupation", "events", "status", "cache_stats")
 
 
 def go_simulate(N, P, nu, kT, I_0, R, distances, E_constant, site_energies, transitions_constant,
@@ -166,14 +166,16 @@ def go_simulate(N, P, nu, kT, I_0, R, distances, E_constant, site_energies, tran
     u = None if uniforms is None else np.ascontiguousarray(uniforms, dtype=np.float64).ravel()
     ev = np.zeros(2 * hops, dtype=np.int32) if log else None
     t = C.c_double(0.0)
+    stats = np.zeros(3, dtype=np.int64)
     st = lib().oracle_go_simulate(
         N, P, float(prune_threshold), nu, kT, I_0, R, _p(d, _dp), _p(E, _dp), _p(tc, _dp),
         _p(se, _dp), _p(occ, _u8p), _p(eo, _dp), int(hops), int(bool(full_recompute)),
         int(bool(use_cache)), int(bool(record)), _p(traffic, _dp), _p(avg, _dp), _p(u, _dp),
-        int(seed), int(hop0), int(cache_max_bytes), _p(ev, _i32p), C.byref(t))
+        int(seed), int(hop0), int(cache_max_bytes), _p(ev, _i32p), C.byref(t), _p(stats, _i64p))
     r = GoResult()
     r.time, r.occupation, r.electrode_occupation, r.site_energies = t.value, occ.astype(bool), eo, se
     r.traffic, r.average_occupation, r.status = traffic, avg, st
+    r.cache_stats = {"hits": int(stats[0]), "misses": int(stats[1]), "states": int(stats[2])}
     r.events = None if ev is None else ev.reshape(hops, 2)
     return r
 
@@ -204,8 +206,12 @@ def numba_ensemble(N, P, nu, kT, I_0, R, distances, E_constant, site_energies, t
 
 def go_ensemble(N, P, nu, kT, I_0, R, distances, E_constant, site_energies, transitions_constant,
                 occupation, seeds, hops, full_recompute=False, use_cache=False,
-                cache_max_bytes=1 << 26, hop0=0, nthreads=0):
-    """Go-restatement trajectories on OpenMP threads; same return layout."""
+                cache_max_bytes=1 << 26, hop0=0, nthreads=0, prune_threshold=0.0,
+                cache_stats=None):
+    """Go-restatement trajectories on OpenMP threads; same return layout.
+
+    ``cache_stats``: optional int64 array (K, 3) that receives each trajectory's
+    cache hits, misses and distinct occupation states."""
     seeds = np.ascontiguousarray(seeds, dtype=np.uint64)
     K = seeds.size
     d = np.ascontiguousarray(distances, dtype=np.float64)
@@ -217,11 +223,15 @@ def go_ensemble(N, P, nu, kT, I_0, R, distances, E_constant, site_energies, tran
     tm = np.zeros(K)
     occ = np.zeros((K, N), dtype=np.uint8)
     st = np.zeros(K, dtype=np.int32)
+    if cache_stats is not None:
+        assert cache_stats.dtype == np.int64 and cache_stats.shape == (K, 3)
+        assert cache_stats.flags.c_contiguous
     lib().oracle_go_ensemble(N, P, nu, kT, I_0, R, _p(d, _dp), _p(E, _dp), _p(se, _dp),
                              _p(tc, _dp), _p(occ0, _u8p), K, _p(seeds, _u64p), int(hop0),
                              int(hops), int(bool(full_recompute)), int(bool(use_cache)),
-                             int(cache_max_bytes), int(nthreads), _p(eo, _dp), _p(tm, _dp),
-                             _p(occ, _u8p), _p(st, _i32p))
+                             int(cache_max_bytes), int(nthreads), float(prune_threshold),
+                             _p(eo, _dp), _p(tm, _dp), _p(occ, _u8p), _p(st, _i32p),
+                             _p(cache_stats, _i64p))
     return eo, tm, occ, st
 
 

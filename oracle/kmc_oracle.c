@@ -325,7 +325,8 @@ int oracle_go_simulate(int64_t NSites, int64_t NElectrodes, double prune_thresho
                        int full_recompute, int use_cache, int record,
                        double* traffic, double* average_occupation,
                        const double* uniforms, uint64_t seed, uint64_t hop0,
-                       int64_t cache_max_bytes, int32_t* events_from_to, double* time_out) {
+                       int64_t cache_max_bytes, int32_t* events_from_to, double* time_out,
+                       int64_t* cache_stats) {
     const int N = (int)(NSites + NElectrodes);
     const int NS = (int)NSites;
     if (NSites < 0 || NElectrodes < 0 || N <= 0 || NS > 256) return ERR_ARGS;
@@ -381,6 +382,7 @@ int oracle_go_simulate(int64_t NSites, int64_t NElectrodes, double prune_thresho
     uint32_t reuse_threshold = 1;
     uint64_t count_storage = 0, threshold_increase = 100000;
     int status = OK;
+    int64_t n_hit = 0, n_miss = 0;
 
     for (int64_t hop = 0; hop < hops; ++hop) {
         const float* pl = NULL;
@@ -390,6 +392,7 @@ int oracle_go_simulate(int64_t NSites, int64_t NElectrodes, double prune_thresho
             for (int i = 0; i < NS; ++i) if (occupation[i]) key[i >> 6] |= 1ull << (i & 63);
             ent = go_cache_find(&cache, key, 0);
             if (ent && ent->slot >= 0) pl = cache.pool + (int64_t)ent->slot * nt;
+            if (pl) ++n_hit; else ++n_miss;
         }
         if (!pl) {
             if (full_recompute) {   /* :378-386 */
@@ -474,6 +477,11 @@ int oracle_go_simulate(int64_t NSites, int64_t NElectrodes, double prune_thresho
         }
     }
     for (int i = 0; i < N; ++i) site_energies_d[i] = se[i];
+    if (cache_stats) {   /* cache hits, misses, distinct occupation states seen */
+        cache_stats[0] = n_hit;
+        cache_stats[1] = n_miss;
+        cache_stats[2] = cache.used;
+    }
     free(cache.tab); free(cache.pool);
     free(d); free(tc); free(E); free(se); free(tf); free(tt); free(rate); free(prob);
     if (time_out) *time_out = time;
@@ -517,13 +525,16 @@ int oracle_numba_ensemble(int64_t N, int64_t P, double nu, double kT, double I_0
     return bad ? ERR_ZERO_TOTAL : OK;
 }
 
+/* prune_threshold as wrapperSimulatePruned (simulationWrapper.go:98-110);
+ * cache_stats (optional): 3 per trajectory, as oracle_go_simulate writes them. */
 int oracle_go_ensemble(int64_t N, int64_t P, double nu, double kT, double I_0, double R,
                        const double* distances, const double* E_constant,
                        const double* site_energies0, const double* transitions_constant,
                        const uint8_t* occupation0, int64_t ntraj, const uint64_t* seeds,
                        uint64_t hop0, int64_t hops, int full_recompute, int use_cache,
-                       int64_t cache_max_bytes, int nthreads,
-                       double* eo_out, double* time_out, uint8_t* occ_out, int32_t* status_out) {
+                       int64_t cache_max_bytes, int nthreads, double prune_threshold,
+                       double* eo_out, double* time_out, uint8_t* occ_out, int32_t* status_out,
+                       int64_t* cache_stats) {
     const int64_t S = N + P;
 #ifdef _OPENMP
     if (nthreads > 0) omp_set_num_threads(nthreads);
@@ -537,10 +548,11 @@ int oracle_go_ensemble(int64_t N, int64_t P, double nu, double kT, double I_0, d
         uint8_t* occ = occ_out + k * N;
         memcpy(se, site_energies0, sizeof(double) * S);
         memcpy(occ, occupation0, (size_t)N);
-        int st = oracle_go_simulate(N, P, 0.0, nu, kT, I_0, R, distances, E_constant,
+        int st = oracle_go_simulate(N, P, prune_threshold, nu, kT, I_0, R, distances, E_constant,
                                     transitions_constant, se, occ, eo_out + k * P, hops,
                                     full_recompute, use_cache, 0, NULL, NULL, NULL, seeds[k], hop0,
-                                    cache_max_bytes, NULL, &time_out[k]);
+                                    cache_max_bytes, NULL, &time_out[k],
+                                    cache_stats ? cache_stats + 3 * k : NULL);
         status_out[k] = st;
         bad |= (st != 0);
         free(se);

@@ -88,3 +88,47 @@ def test_parallel_simulations_like_parrallelsimulationbind(gpu_engine):
     assert np.all(times > 0)
     eo = _read(slices["eo"], sum(cols["P"]))
     assert np.all(np.isfinite(eo)) and np.abs(eo).sum() > 0
+
+
+def test_wrapper_simulate_record_plus_like_pythonbind(fixtures, gpu_engine):
+    """wrapperSimulateRecordPlus (simulationWrapper.go:149-169) called as
+    pythonBind.py:64-81 calls it, repeatedly, on a reference fixture network:
+    the currents of the calls agree within 3 sigma with the oracle's
+    restatement of simulateRecordPlus (simulation.go:327-432: empty start,
+    full FP32 recompute on a miss, occupation cache) on the same network."""
+    import oracle
+    from tests.netgen import fixture_network
+    from tests.stats import assert_within, compare
+    net = fixture_network(fixtures, "rnd", 11)
+    N, P = net["N"], net["P"]
+    lib = C.CDLL(_lib.LIB_PATH)
+    f = lib.wrapperSimulateRecordPlus
+    f.argtypes = [C.c_longlong, C.c_longlong, C.c_double, C.c_double, C.c_double, C.c_double,
+                  C.c_double, GoSlice, GoSlice, GoSlice, GoSlice, GoSlice, GoSlice, C.c_int,
+                  C.c_bool, GoSlice, GoSlice]
+    f.restype = C.c_double
+    keep = []
+
+    def gs(x):
+        s, a = go_slice(np.ravel(x))
+        keep.append(a)
+        return s
+    calls, hops = 48, 50_000
+    cur = np.zeros((calls, P))
+    d, E, tc = gs(net["distances"]), gs(net["E_constant"]), gs(net["transitions_constant"])
+    se = gs(net["site_energies"])
+    print(f"[gpu] wrapperSimulateRecordPlus: {calls} calls x {hops} hops", flush=True)
+    for i in range(calls):
+        eo = gs(np.zeros(P))
+        t = f(N, P, 1.0, 1.0, net["I_0"], net["R"], 0.0, gs(net["occupation"].astype(float)), d,
+              E, tc, eo, se, hops, False, gs(np.zeros((N + P) ** 2)), gs(np.zeros(N)))
+        assert np.isfinite(t) and t > 0, _lib.load().kmc_last_error()
+        cur[i] = _read(eo, P) / t
+    eo, tm, _, st = oracle.go_ensemble(
+        N, P, net["nu"], net["kT"], net["I_0"], net["R"], net["distances"], net["E_constant"],
+        net["site_energies"], net["transitions_constant"], np.zeros(N, np.uint8),
+        np.arange(calls, dtype=np.uint64), hops, full_recompute=True, use_cache=True)
+    assert (st == 0).all()
+    c = compare(cur, eo / tm[:, None])
+    assert c["resolvable"].sum() >= 4
+    assert_within(c, "wrapperSimulateRecordPlus vs the simulateRecordPlus restatement")

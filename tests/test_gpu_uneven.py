@@ -4,6 +4,8 @@ import os
 import numpy as np
 import pytest
 
+from tests.stats import assert_within, compare
+
 from kmc_dn_amd import _lib, autotune, engine, uneven, workloads
 
 pytestmark = pytest.mark.gpu
@@ -93,18 +95,10 @@ def test_mixed_forms_statistics(gpu_engine):
     finally:
         for lay in lays:
             lay.close()
-    zs = []
-    for i, net in enumerate(nets):
-        sl = slice(i * reps, (i + 1) * reps)
-        a = out["pos"].currents(net.P)[sl]
-        c = out["tab"].currents(net.P)[sl]
-        se = np.sqrt(a.var(0, ddof=1) / reps + c.var(0, ddof=1) / reps)
-        ok = se > 0
-        zs.append(np.abs(a.mean(0) - c.mean(0))[ok] / se[ok])
-    z = np.concatenate(zs)
-    print(f"[gpu] positions vs tables over {len(nets)} networks: median |z| {np.median(z):.2f}, "
-          f"max {z.max():.2f}", flush=True)
-    assert np.median(z) < 1.2 and z.max() < 5.0
+    cs = [compare(out["pos"].currents(net.P)[i * reps:(i + 1) * reps],
+                  out["tab"].currents(net.P)[i * reps:(i + 1) * reps]) for i, net in enumerate(nets)]
+    assert_within({k: np.concatenate([c[k] for c in cs]) for k in ("z", "resolvable", "rel")},
+                  f"positions vs tables over {len(nets)} networks")
 
 
 def test_uneven_fp32_vs_fp64(gpu_engine):
@@ -129,14 +123,9 @@ def test_uneven_fp32_vs_fp64(gpu_engine):
             assert (r.status == 0).all()
             cur[prec] = r.currents(net.P)
         lay.close()
-        a, c = cur[_lib.FP64_DENSE], cur[_lib.FP32_BATCHED]
-        se = np.sqrt(a.var(0, ddof=1) / reps + c.var(0, ddof=1) / reps)
-        ok = se > 0
-        zs.append(np.abs(a.mean(0) - c.mean(0))[ok] / se[ok])
-    z = np.concatenate(zs)
-    print(f"[gpu] fp32 positions vs fp64: median |z| {np.median(z):.2f}, max {z.max():.2f}",
-          flush=True)
-    assert np.median(z) < 1.2 and z.max() < 5.0
+        zs.append(compare(cur[_lib.FP32_BATCHED], cur[_lib.FP64_DENSE]))
+    assert_within({k: np.concatenate([c[k] for c in zs]) for k in ("z", "resolvable", "rel")},
+                  "fp32 positions vs fp64 over 6 uneven networks")
 
 
 def test_autotune_plan_and_cache(small_uneven, tmp_path):
