@@ -3,9 +3,11 @@
 // Restates _simulate_discrete_record (kmc_dopant_networks.py:33-135) with the
 // same FP64 operations in the same order, so that with a replayed uniform
 // stream the event sequence equals the CPU restatement's bit for bit:
-//   * site energies: se[i] = E[i] + ((-I_0)*R) * sum_{j != i, j empty} 1/d[i][j],
-//     summed over ascending j (:59-66); 1/d is precomputed (same IEEE quotient)
-//     and an occupied j adds an exact +0.0, so the partial sums are unchanged;
+//   * site energies: se[i] = E[i] + ((-I_0)*R) * sum_{j != i} (1 - n_j)/d[i][j],
+//     summed over ascending j (:59-66).  An occupied j adds an exact +0.0 to a
+//     sum that is +0.0 or positive, which leaves it unchanged, so only the
+//     empty acceptors are visited (ascending); 1/d is precomputed (the same
+//     IEEE quotient) with a zero diagonal;
 //   * rates: tc[i][j] * (dE > 0 ? nu*exp(-dE/kT) : nu) for possible pairs (:69-87);
 //   * prefix sum in row-major order (:91-97).  Impossible pairs contribute an
 //     exact 0.0, so the sequential chain runs over the possible pairs only, in
@@ -14,38 +16,56 @@
 //     (:100-110); u == 0 selects k = 0 exactly as the reference's scan does.
 // This file is compiled with -ffp-contract=off: no a*b+c is fused.
 //
-// One wave (64 lanes) per trajectory.  The pair loop, rate evaluation and the
-// final search are lane-parallel; only the prefix-sum chain is serial, run by
-// one lane eight values at a time from registers.  With STAGE the layout's
+// One wave (64 lanes) per trajectory.  The possible pairs of a hop have a
+// closed structure (_transition_possible, :137-163): rows of occupied acceptors
+// (ascending) hop into [empty acceptors ascending, then the electrodes], and
+// then each electrode row hops into the empty acceptors.  Two short LDS lists
+// (occupied acceptors; empty acceptors followed by electrodes) built with
+// ballots give every list position its (row, column) by one small division,
+// so the rates are lane-parallel over the compacted list.  Only the prefix
+// chain is serial: one lane runs it from registers, 16 values per LDS block
+// with the next block's loads issued before the current block's adds.  The
+// event search is monotone (the prefix never decreases and the quotient by
+// the total is monotone), so two lane-parallel rounds find the first index
+// (a coarse pass over 64 block ends, then the block).  With STAGE the layout's
 // tables (1/d, tc, (I_0*R)/d) are copied to LDS once per trajectory.
 #include "kmc_device.h"
 
 namespace kmc {
 
-// _transition_possible, kmc_dopant_networks.py:137-163
-__device__ __forceinline__ bool possible(int i, int j, int N, const Occ4& o) {
-  if (i >= N && j >= N) return false;
-  if (i >= N) return !o.test(j);
-  if (j >= N) return o.test(i);
-  if (i == j) return false;
-  return o.test(i) && !o.test(j);
-}
-
 // LDS carve-up for S sites, N acceptors (all offsets in doubles, 16-byte aligned).
 struct PlainLds {
-  size_t se, buf, list, invd, tc, ird, total;
+  size_t se, buf, lists, invd, tc, ird, total;
   __host__ __device__ PlainLds(int S, int N, bool stage) {
     auto up2 = [](size_t x) { return (x + 1) & ~(size_t)1; };
     const size_t SS = (size_t)S * S;
     se = 0;
     buf = up2(S);
-    list = buf + up2(SS);                       // int16 entries, SS/4 doubles
-    invd = list + up2((SS * sizeof(int16_t) + 7) / 8);
+    // possible pairs: nA (N - nA + P) + P (N - nA) <= N^2/4 + P N for nA occupied
+    const size_t npairs = (size_t)N * N / 4 + (size_t)(S - N) * N + 16;
+    lists = buf + up2(npairs);                  // int16: occupied acceptors [N], targets [S]
+    invd = lists + up2(((size_t)N + S + 3) / 4);
     tc = invd + (stage ? up2((size_t)N * N) : 0);
     ird = tc + (stage ? up2(SS) : 0);
     total = ird + (stage ? up2(SS) : 0);
   }
 };
+
+// q = x / d and r = x - q*d for 0 <= x < 2^20, 1 <= d <= 256: an FP32 estimate
+// from the reciprocal (relative error ~1e-7, so q is off by at most one), fixed.
+__device__ __forceinline__ int div_small(int x, int d, float rcp, int& r) {
+  int q = (int)((float)x * rcp);
+  r = x - q * d;
+  if (r < 0) {
+    q -= 1;
+    r += d;
+  }
+  if (r >= d) {
+    q += 1;
+    r -= d;
+  }
+  return q;
+}
 
 template <bool REC, bool STAGE>
 __global__ void __launch_bounds__(64) kmc_plain_fp64_kernel(PlainParams p) {
@@ -55,15 +75,19 @@ __global__ void __launch_bounds__(64) kmc_plain_fp64_kernel(PlainParams p) {
   const TrajDev td = p.trajs[t];
   const LayoutDev L = p.layouts[td.layout];
   const EsetDev es = p.esets[td.eset];
-  const int N = L.N, S = L.S, SS = S * S;
+  const int N = L.N, P = L.P, S = L.S, SS = S * S;
   const double nu = L.nu, kT = L.kT;
   const double nIR = (-L.I0) * L.R;
+  // kT == 1 and nu == 1 make -dE/kT and nu*exp(x) exact copies of their
+  // operand: the same bits with or without the operation
+  const bool kT1 = kT == 1.0, nu1 = nu == 1.0;
 
   double* dl = reinterpret_cast<double*>(smem);
   const PlainLds off(S, N, STAGE);
   double* se = dl + off.se;
   double* buf = dl + off.buf;     // rates of the possible pairs, then their prefix sums
-  int16_t* list = reinterpret_cast<int16_t*>(dl + off.list);
+  int16_t* srcA = reinterpret_cast<int16_t*>(dl + off.lists);   // occupied acceptors
+  int16_t* tgt = srcA + N;        // empty acceptors, then electrodes N..S-1
   const double* invd = L.invd64;
   const double* tcp = L.tc64;
   const double* irdp = L.ird64;
@@ -82,12 +106,15 @@ __global__ void __launch_bounds__(64) kmc_plain_fp64_kernel(PlainParams p) {
   }
 
   Occ4 occ{td.occ[0], td.occ[1], td.occ[2], td.occ[3]};
+  // lanes of site chunk c (site lane + 64c) holding acceptors; S <= 80 < 128
+  const uint64_t acc0 = N >= 64 ? ~0ull : ((1ull << N) - 1);
+  const uint64_t acc1 = N <= 64 ? 0ull : ((1ull << (N - 64)) - 1);
 
-  for (int e = lane; e < L.P; e += 64) se[N + e] = es.V[e];
+  for (int e = lane; e < P; e += 64) se[N + e] = es.V[e];
   double E0 = 0.0, E1 = 0.0;
   if (lane < N) E0 = es.E[lane];
   if (lane + 64 < N) E1 = es.E[lane + 64];
-  __syncthreads();
+  wave_lds_sync();
   int64_t eo = 0;                 // lane e < P holds electrode e's tally
   double ot0 = 0.0, ot1 = 0.0;    // post-hop occupation time, sites lane and lane+64
   double time = 0.0;
@@ -101,62 +128,106 @@ __global__ void __launch_bounds__(64) kmc_plain_fp64_kernel(PlainParams p) {
       eo = 0;
       ot0 = ot1 = 0.0;
     }
-    // site energies, :59-66
+    // occupied / empty acceptors as wave-uniform masks of the two site chunks
+    const uint64_t o0 = occ.a & acc0, o1 = occ.b & acc1;
+    const uint64_t e0 = acc0 & ~o0, e1 = acc1 & ~o1;
+    // site energies, :59-66: the empty acceptors j in ascending order
+    {
+      double a0 = 0.0, a1 = 0.0;
 #pragma unroll
-    for (int part = 0; part < 2; ++part) {
-      const int i = lane + 64 * part;
-      if (i < N) {
-        double acc = 0.0;
-        for (int j = 0; j < N; ++j) {
-          const bool add = !occ.test(j) && j != i;
-          const double x = invd[j * N + i];
-          acc += add ? x : 0.0;
+      for (int w = 0; w < 2; ++w) {
+        uint64_t m = w ? e1 : e0;
+        while (m) {
+          const int b = __builtin_ctzll(m);
+          m &= m - 1;
+          const int j = b + 64 * w;
+          if (lane < N) a0 += invd[j * N + lane];
+          if (lane + 64 < N) a1 += invd[j * N + lane + 64];
         }
-        se[i] = (part ? E1 : E0) + nIR * acc;
+      }
+      if (lane < N) se[lane] = E0 + nIR * a0;
+      if (lane + 64 < N) se[lane + 64] = E1 + nIR * a1;
+    }
+    // the lists: occupied acceptors, then targets (empty acceptors, electrodes)
+    const int nA = __builtin_popcountll(o0) + __builtin_popcountll(o1);
+    const int cE = N - nA;
+    const int cA = cE + P;
+    {
+      const bool oc0 = (o0 >> lane) & 1ull, oc1 = (o1 >> lane) & 1ull;
+      const bool em0 = (e0 >> lane) & 1ull, em1 = (e1 >> lane) & 1ull;
+      const int lo_o = __builtin_popcountll(o0), lo_e = __builtin_popcountll(e0);
+      if (oc0) srcA[mbcnt(o0)] = (int16_t)lane;
+      if (oc1) srcA[lo_o + mbcnt(o1)] = (int16_t)(lane + 64);
+      if (em0) tgt[mbcnt(e0)] = (int16_t)lane;
+      if (em1) tgt[lo_e + mbcnt(e1)] = (int16_t)(lane + 64);
+      if (lane < P) tgt[cE + lane] = (int16_t)(N + lane);
+    }
+    const int nAA = nA * cA;
+    const int n = nAA + P * cE;
+    wave_lds_sync();
+    // rates of the possible pairs in row-major order, :69-87
+    const float rA = cA > 0 ? __builtin_amdgcn_rcpf((float)cA) : 0.f;
+    const float rE = cE > 0 ? __builtin_amdgcn_rcpf((float)cE) : 0.f;
+    for (int b0 = 0; b0 < n; b0 += 64) {
+      const int idx = b0 + lane;
+      if (idx < n) {
+        int i, j, r;
+        if (idx < nAA) {
+          const int q = div_small(idx, cA, rA, r);
+          i = srcA[q];
+          j = tgt[r];
+        } else {
+          const int q = div_small(idx - nAA, cE, rE, r);
+          i = N + q;
+          j = tgt[r];
+        }
+        const int k = i * S + j;
+        double dE;
+        if (i < N && j < N) dE = se[j] - se[i] - irdp[k];
+        else dE = se[j] - se[i];
+        double m;
+        if (dE > 0) {
+          const double x = kT1 ? -dE : -dE / kT;
+          const double ex = exp(x);
+          m = nu1 ? ex : nu * ex;
+        } else {
+          m = nu;
+        }
+        buf[idx] = tcp[k] * m;
       }
     }
-    __syncthreads();
-    // possible pairs, compacted in row-major order
-    int n = 0;
-    for (int k0 = 0; k0 < SS; k0 += 64) {
-      const int k = k0 + lane;
-      const int i = k / S, j = k - (k / S) * S;
-      const bool ok = k < SS && possible(i, j, N, occ);
-      const uint64_t m = ballot(ok);
-      if (ok) list[n + mbcnt(m)] = (int16_t)k;
-      n += __builtin_popcountll(m);
-    }
-    __syncthreads();
-    // rates of the possible pairs, :69-87
-    for (int idx = lane; idx < n; idx += 64) {
-      const int k = list[idx];
-      const int i = k / S, j = k - (k / S) * S;
-      double dE;
-      if (i < N && j < N) dE = se[j] - se[i] - irdp[k];
-      else dE = se[j] - se[i];
-      double m;
-      if (dE > 0) m = nu * exp(-dE / kT);
-      else m = nu;
-      buf[idx] = tcp[k] * m;
-    }
-    __syncthreads();
-    // cumulative sum, :91-97: one lane, eight values per LDS round trip
+    wave_lds_sync();
+    // cumulative sum, :91-97: one lane, 16 values per block, the next block's
+    // loads in flight during this block's adds
     double acc = 0.0;
     if (lane == 0) {
-      int idx = 0;
-      for (; idx + 8 <= n; idx += 8) {
-        double r[8];
+      const double2* src = reinterpret_cast<const double2*>(buf);
+      double2* dst = reinterpret_cast<double2*>(buf);
+      const int nb = n >> 4;
+      double2 cur[8], nxt[8];
+      if (nb > 0) {
 #pragma unroll
-        for (int q = 0; q < 8; ++q) r[q] = buf[idx + q];
+        for (int q = 0; q < 8; ++q) cur[q] = src[q];
+      }
+#pragma unroll 1
+      for (int blk = 0; blk < nb; ++blk) {
+        if (blk + 1 < nb) {
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
-          acc = r[q] + acc;
-          r[q] = acc;
+          for (int q = 0; q < 8; ++q) nxt[q] = src[(blk + 1) * 8 + q];
         }
 #pragma unroll
-        for (int q = 0; q < 8; ++q) buf[idx + q] = r[q];
+        for (int q = 0; q < 8; ++q) {
+          acc = cur[q].x + acc;
+          cur[q].x = acc;
+          acc = cur[q].y + acc;
+          cur[q].y = acc;
+        }
+#pragma unroll
+        for (int q = 0; q < 8; ++q) dst[blk * 8 + q] = cur[q];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) cur[q] = nxt[q];
       }
-      for (; idx < n; ++idx) {
+      for (int idx = nb << 4; idx < n; ++idx) {
         acc = buf[idx] + acc;
         buf[idx] = acc;
       }
@@ -176,21 +247,34 @@ __global__ void __launch_bounds__(64) kmc_plain_fp64_kernel(PlainParams p) {
     }
     const double scale = 1.0 / total;
     const double hop_time = -log(1.0 - uc) * scale;
-    __syncthreads();
-    // first k with problist[k]/total >= u, :103-110
+    wave_lds_sync();
+    // first k with problist[k]/total >= u, :103-110 (monotone in k)
     int k_ev = 0;
     if (ue != 0.0) {
+      const int stride = (n + 63) >> 6;
+      const int end = min(n, (lane + 1) * stride) - 1;
+      const bool hit = lane * stride < n && buf[end] / total >= ue;
+      const uint64_t mc = ballot(hit);
       int found = -1;
-      for (int b = 0; b < n; b += 64) {
-        const int idx = b + lane;
-        const bool hit = idx < n && buf[idx] / total >= ue;
-        const uint64_t m = ballot(hit);
-        if (m) {
-          found = b + __builtin_ctzll(m);
-          break;
+      if (mc) {
+        const int base = __builtin_ctzll(mc) * stride;
+        const int pos = base + lane;
+        const uint64_t mf = ballot(lane < stride && pos < n && buf[pos] / total >= ue);
+        found = base + __builtin_ctzll(mf);   // the block's last entry is a hit
+      }
+      if (found >= 0) {
+        int r;
+        if (found < nAA) {
+          const int q = div_small(found, cA, rA, r);
+          k_ev = srcA[q] * S + tgt[r];
+        } else {
+          const int q = div_small(found - nAA, cE, rE, r);
+          k_ev = (N + q) * S + tgt[r];
         }
       }
-      k_ev = found >= 0 ? (int)list[found] : 0;
+      // the same on every lane: keep the event and the occupation it updates
+      // on the scalar unit
+      k_ev = __builtin_amdgcn_readfirstlane(k_ev);
     }
     const int from = k_ev / S, to = k_ev - (k_ev / S) * S;
     if (p.events && t == 0 && lane == 0) p.events[h] = k_ev;
@@ -206,7 +290,7 @@ __global__ void __launch_bounds__(64) kmc_plain_fp64_kernel(PlainParams p) {
       if (lane + 64 < N && occ.test(lane + 64)) ot1 += hop_time;
     }
     time += hop_time;
-    __syncthreads();
+    wave_lds_sync();
   }
   TrajOut* o = p.out + t;
   if (lane == 0) {
