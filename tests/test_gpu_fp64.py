@@ -108,3 +108,29 @@ def test_zero_rate_is_reported(gpu_engine):
                                  net["E_constant"], se, net["transitions_constant"], occ,
                                  np.zeros(0, np.int64), 3)
     assert e.value.code == _lib.ERR_ZERO_RATE
+
+
+def test_zero_event_uniform_selects_the_first_pair(gpu_engine):
+    """u == 0 passes the reference's `problist[i] >= event` test at i = 0
+    (kmc_dopant_networks.py:106-110): the (0, 0) self-pair, which empties and
+    refills acceptor 0, so acceptor 0 ends occupied.  The FP64 kernel keeps
+    that quirk hop for hop with the oracle."""
+    net = random_network(9, 20, 6, I_0=30.0, a=0.3, vmax=2.0)
+    N, P, hops = 20, 6, 400
+    U = np.random.RandomState(3).rand(2 * hops)
+    zero_hops = np.arange(5, hops, 37)
+    U[2 * zero_hops + 1] = 0.0
+    ref = oracle.numba_simulate(N, P, net["nu"], net["kT"], net["I_0"], net["R"],
+                                net["distances"], net["E_constant"], net["site_energies"],
+                                net["transitions_constant"], net["occupation"],
+                                np.zeros(P, np.int64), hops, uniforms=U, log=True)
+    se = net["site_energies"].copy()
+    occ = net["occupation"].copy()
+    eo = np.zeros(P, np.int64)
+    print("[gpu] u == 0 hops on the FP64 kernel", flush=True)
+    t, _, _, ev, _ = engine.simulate_discrete(
+        N, P, net["nu"], net["kT"], net["I_0"], net["R"], net["distances"], net["E_constant"], se,
+        net["transitions_constant"], occ, eo, hops, uniforms=U, log=True)
+    assert (ev[zero_hops] == 0).all() and (ref.events[zero_hops] == 0).all()
+    assert np.array_equal(ev, ref.events)
+    assert np.array_equal(occ, ref.occupation) and np.array_equal(eo, ref.electrode_occupation)
