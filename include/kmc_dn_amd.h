@@ -86,6 +86,30 @@ int kmc_simulate_discrete(int64_t N, int64_t P, double nu, double kT, double I_0
                           int32_t* events_out, double* totals_out, double* time_out);
 
 /*
+ * No counterpart in the reference: kmc_simulate_discrete for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Same arguments, kernel and results;
+ * it calls no hipMalloc/hipFree and no allocating routine: every device buffer
+ * is carved from one static scratch array per device (about 2.1 MiB, held
+ * until the process ends).  Limits: S = N + P <= KMC_NOALLOC_MAX_SITES, and
+ * with replayed uniforms or event/total logs at most KMC_NOALLOC_MAX_LOGGED_HOPS
+ * hops (KMC_ERR_LIMIT beyond).  Calls on one device share its scratch and keep
+ * the last call's contents; the engine lock serialises the calls of a process,
+ * and calls that overlap on one device (bypassing it) would race.  The call is
+ * synchronous: on return every output is written.
+ */
+#define KMC_NOALLOC_MAX_SITES 80
+#define KMC_NOALLOC_MAX_LOGGED_HOPS 65536
+int kmc_simulate_discrete_noalloc(int64_t N, int64_t P, double nu, double kT, double I_0,
+                                  double R, const double* distances, const double* E_constant,
+                                  double* site_energies, const double* transitions_constant,
+                                  uint8_t* occupation, int64_t* electrode_occupation,
+                                  int64_t hops, int record, double* traffic,
+                                  double* occupations_in_time, const double* uniforms,
+                                  uint64_t seed, uint64_t hop0, int32_t* events_out,
+                                  double* totals_out, double* time_out);
+
+/*
  * probSimulate (goSimulation/probabilitySimulation.go:53-157): the Go library's
  * deterministic mean-field integrator, `steps` steps of n networks at once
  * (network i: layout layouts[i], energy set esets[i] of that layout), FP64.

@@ -71,6 +71,9 @@ SIGNATURES = {
     "kmc_simulate_discrete": (C.c_int, [
         _i64, _i64, C.c_double, C.c_double, C.c_double, C.c_double, _dp, _dp, _dp, _dp, _u8p,
         _i64p, _i64, C.c_int, _dp, _dp, _dp, C.c_uint64, C.c_uint64, _i32p, _dp, _dp]),
+    "kmc_simulate_discrete_noalloc": (C.c_int, [
+        _i64, _i64, C.c_double, C.c_double, C.c_double, C.c_double, _dp, _dp, _dp, _dp, _u8p,
+        _i64p, _i64, C.c_int, _dp, _dp, _dp, C.c_uint64, C.c_uint64, _i32p, _dp, _dp]),
     "kmc_meanfield_simulate": (C.c_int, [_i32, _i32p, _i32p, _i64, _dp, _dp, _dp, _dp, _dp, _dp]),
     "kmc_average_current": (C.c_int, [_i32, _i32p, _i32p, _dp, _dp]),
     "kmc_layout_create": (C.c_int, [_i64, _i64, C.c_double, C.c_double, C.c_double, C.c_double,

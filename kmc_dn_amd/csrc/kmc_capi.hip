@@ -41,9 +41,66 @@ __global__ void kmc_meanfield_kernel(MeanfieldParams);
 template <int C>
 __global__ void kmc_avg_current_kernel(MeanfieldParams);
 
+// Device memory of kmc_simulate_discrete_noalloc, the entry point for callers
+// that cannot allocate: every buffer its kernels use (descriptors, the layout's
+// FP64 tables, record buffers, replayed uniforms and event logs) is carved
+// from this one file-scope array, so that entry point calls no hipMalloc.  Each
+// device holds its own copy for the life of the process; host code uses the
+// address hipGetSymbolAddress gives for it (noalloc_scratch, looked up once per
+// device when the engine is initialised), never the array's name.  Every call
+// on a device shares that device's copy and leaves its contents behind; calls
+// of one process are serialised by the engine lock, so two calls never overlap
+// on it (callers that bypassed the lock would race).
+constexpr size_t NOALLOC_TABLE = (size_t)KMC_NOALLOC_MAX_SITES * KMC_NOALLOC_MAX_SITES * sizeof(double);
+constexpr size_t NOALLOC_ALIGN = 256;
+constexpr size_t noalloc_up(size_t x) { return (x + NOALLOC_ALIGN - 1) & ~(NOALLOC_ALIGN - 1); }
+// carve-up (byte offsets): descriptors, E, V, d, tc, ird, invd, traffic,
+// se_out, occ_time, uniforms, events, totals
+constexpr size_t NA_DESC = 0;
+constexpr size_t NA_E = noalloc_up(NA_DESC + 1024);
+constexpr size_t NA_V = noalloc_up(NA_E + sizeof(double) * KMC_NOALLOC_MAX_SITES);
+constexpr size_t NA_D = noalloc_up(NA_V + sizeof(double) * KMC_PMAX);
+constexpr size_t NA_TC = NA_D + NOALLOC_TABLE;
+constexpr size_t NA_IRD = NA_TC + NOALLOC_TABLE;
+constexpr size_t NA_INVD = NA_IRD + NOALLOC_TABLE;
+constexpr size_t NA_TRAFFIC = NA_INVD + NOALLOC_TABLE;
+constexpr size_t NA_SE = NA_TRAFFIC + NOALLOC_TABLE;
+constexpr size_t NA_OT = noalloc_up(NA_SE + sizeof(double) * KMC_NMAX);
+constexpr size_t NA_U = noalloc_up(NA_OT + sizeof(double) * KMC_NMAX);
+constexpr size_t NA_EV = NA_U + sizeof(double) * 2 * KMC_NOALLOC_MAX_LOGGED_HOPS;
+constexpr size_t NA_TOT = noalloc_up(NA_EV + sizeof(int32_t) * KMC_NOALLOC_MAX_LOGGED_HOPS);
+constexpr size_t NOALLOC_BYTES = noalloc_up(NA_TOT + sizeof(double) * KMC_NOALLOC_MAX_LOGGED_HOPS);
+static_assert(sizeof(LayoutDev) + sizeof(EsetDev) + sizeof(TrajDev) + sizeof(TrajOut) + 64 <= 1024,
+              "descriptor block");
+static_assert(NOALLOC_TABLE % NOALLOC_ALIGN == 0, "table alignment");
+
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+// The layout tables kmc_layout_create builds on the host, built in scratch:
+// ird[i*S+j] = (I_0*R)/d[i][j] and invd[j*N+i] = 1/d[i][j] for acceptor pairs
+// i != j (0 elsewhere; the same IEEE quotients as the host's), and the record
+// buffers the kernel accumulates into set to zero.
+__global__ void kmc_noalloc_prep_kernel(const double* d, double* ird, double* invd, double* traffic,
+                                        double* occ_time, int N, int S, double I0R, int clear) {
+  const int SS = S * S;
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < SS; k += gridDim.x * blockDim.x) {
+    const int i = k / S, j = k - (k / S) * S;
+    const bool acc = i < N && j < N && i != j;
+    ird[k] = acc ? I0R / d[k] : 0.0;
+    if (i < N && j < N) invd[j * N + i] = acc ? 1.0 / d[k] : 0.0;
+    if (clear) traffic[k] = 0.0;
+  }
+  if (clear)
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < NMAX; k += gridDim.x * blockDim.x)
+      occ_time[k] = 0.0;
+}
+
 namespace {
 
 thread_local std::string g_err;
+
+// Device address of `scratch` on each device (set when the engine initialises).
+void* noalloc_scratch[64] = {};
 
 int fail(int code, const char* fmt, ...) {
   char buf[512];
@@ -185,6 +242,14 @@ int ensure_init() {
   E.lds_bytes = (int)prop.sharedMemPerBlock;
   snprintf(E.name, sizeof E.name, "%s", prop.gcnArchName);
   HIPOK(hipStreamCreateWithFlags(&E.stream, hipStreamNonBlocking));
+  // the allocation-free entry point's scratch: its address on this device
+  if (dev < 0 || dev >= 64) return fail(KMC_ERR_DEVICE, "device index %d out of range", dev);
+  if (!noalloc_scratch[dev]) {
+    void* sp = nullptr;
+    HIPOK(hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch)));
+    if (!sp) return fail(KMC_ERR_DEVICE, "no device address for the scratch array");
+    noalloc_scratch[dev] = sp;
+  }
   E.ready = true;
   return KMC_OK;
 }
@@ -223,16 +288,35 @@ int upload(DevBuf& b, const std::vector<T>& v) {
   return KMC_OK;
 }
 
-int layout_create_locked(int64_t N, int64_t P, double nu, double kT, double I_0, double R,
-                         const double* d, const double* tc, double prune, int32_t* out) {
+// Sizes, constants and tables of one network (no allocation): S = N + P <=
+// smax, kT > 0, nu > 0, a symmetric transitions_constant, distinct acceptors.
+int check_network(int64_t N, int64_t P, int smax, double nu, double kT, double I_0, double R,
+                  const double* d, const double* tc) {
   if (N < 0 || P < 0 || N > NMAX || P > PMAX || N + P <= 0)
     return fail(KMC_ERR_LIMIT, "network N=%lld P=%lld outside 0<=N<=%d, 0<=P<=%d",
                 (long long)N, (long long)P, NMAX, PMAX);
-  if (N + P > SMAX_FP32)
-    return fail(KMC_ERR_LIMIT, "S=N+P=%lld exceeds %d sites", (long long)(N + P), SMAX_FP32);
-  if (!d || !tc || !out) return fail(KMC_ERR_ARGS, "null table pointer");
+  if (N + P > smax)
+    return fail(KMC_ERR_LIMIT, "S=N+P=%lld exceeds %d sites", (long long)(N + P), smax);
+  if (!d || !tc) return fail(KMC_ERR_ARGS, "null table pointer");
   if (!(kT > 0) || !(nu > 0) || !std::isfinite(I_0) || !std::isfinite(R))
     return fail(KMC_ERR_ARGS, "need kT > 0, nu > 0 and finite I_0, R");
+  const int S = (int)(N + P);
+  for (int i = 0; i < S; ++i)
+    for (int j = 0; j < i; ++j) {
+      const double a = tc[i * S + j], b = tc[j * S + i];
+      if (std::fabs(a - b) > 1e-12 * std::max(std::fabs(a), std::fabs(b)))
+        return fail(KMC_ERR_ARGS, "transitions_constant is not symmetric at (%d,%d)", i, j);
+      if (i < N && j < N && !(d[i * S + j] > 0))
+        return fail(KMC_ERR_ARGS, "acceptors %d and %d coincide (distance %g)", i, j, d[i * S + j]);
+    }
+  return KMC_OK;
+}
+
+int layout_create_locked(int64_t N, int64_t P, double nu, double kT, double I_0, double R,
+                         const double* d, const double* tc, double prune, int32_t* out) {
+  int st0 = check_network(N, P, SMAX_FP32, nu, kT, I_0, R, d, tc);
+  if (st0) return st0;
+  if (!out) return fail(KMC_ERR_ARGS, "null layout id pointer");
   const int S = (int)(N + P);
   const int n = (int)N;
   const double I0R = I_0 * R;
@@ -240,14 +324,6 @@ int layout_create_locked(int64_t N, int64_t P, double nu, double kT, double I_0,
   double tcmax = 0.0;
   for (int k = 0; k < S * S; ++k) tcmax = std::max(tcmax, tc[k]);
   const double cut = prune > 0 ? prune * tcmax : -1.0;
-  for (int i = 0; i < S; ++i)
-    for (int j = 0; j < i; ++j) {
-      const double a = tc[i * S + j], b = tc[j * S + i];
-      if (std::fabs(a - b) > 1e-12 * std::max(std::fabs(a), std::fabs(b)))
-        return fail(KMC_ERR_ARGS, "transitions_constant is not symmetric at (%d,%d)", i, j);
-      if (i < n && j < n && !(d[i * S + j] > 0))
-        return fail(KMC_ERR_ARGS, "acceptors %d and %d coincide (distance %g)", i, j, d[i * S + j]);
-    }
   std::vector<float2> tab((size_t)S * S + 64 * 3), tabB((size_t)S * S + 64 * 3);
   std::vector<float> ccol((size_t)n * S + 64 * 3, 0.f);
   for (auto& x : tab) x = make_float2(-INFINITY, 0.f);
@@ -1003,6 +1079,22 @@ int launch_fp32(Batch* B, const kmc_run_opts& o, const std::vector<Fp32Launch>& 
   return KMC_OK;
 }
 
+// The FP64 dense kernel over ntraj trajectories (one wave each) whose layouts
+// have at most S sites and N acceptors: the layout's tables staged in LDS when
+// they fit.
+int launch_plain(PlainParams p, int S, int N, bool rec, int64_t ntraj, hipStream_t s) {
+  const size_t cap = (size_t)160 * 1024;
+  const bool stage = plain_lds_bytes(S, N, true) <= cap;
+  const size_t lds = plain_lds_bytes(S, N, stage);
+  if (lds > cap) return fail(KMC_ERR_LIMIT, "FP64 kernel needs %zu bytes of LDS", lds);
+  void (*k)(PlainParams) = rec ? (stage ? kmc_plain_fp64_kernel<true, true> : kmc_plain_fp64_kernel<true, false>)
+                               : (stage ? kmc_plain_fp64_kernel<false, true> : kmc_plain_fp64_kernel<false, false>);
+  HIPOK(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  void* args[] = {&p};
+  HIPOK(hipLaunchKernel((const void*)k, dim3((unsigned)ntraj), dim3(64), args, lds, s));
+  return KMC_OK;
+}
+
 int enqueue_fp64(Batch* B, const kmc_run_opts& o, hipStream_t s, const double* uniforms,
                  int32_t* events, double* totals, double* traffic, double* se_out) {
   Engine& E = eng();
@@ -1013,13 +1105,6 @@ int enqueue_fp64(Batch* B, const kmc_run_opts& o, hipStream_t s, const double* u
     int st = ensure_occ_time(B);
     if (st) return st;
   }
-  const size_t cap = (size_t)160 * 1024;
-  const bool stage = plain_lds_bytes(B->smax, B->nmax, true) <= cap;
-  const size_t lds = plain_lds_bytes(B->smax, B->nmax, stage);
-  if (lds > cap) return fail(KMC_ERR_LIMIT, "FP64 kernel needs %zu bytes of LDS", lds);
-  void (*k)(PlainParams) = rec ? (stage ? kmc_plain_fp64_kernel<true, true> : kmc_plain_fp64_kernel<true, false>)
-                               : (stage ? kmc_plain_fp64_kernel<false, true> : kmc_plain_fp64_kernel<false, false>);
-  HIPOK(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   PlainParams p{};
   p.trajs = B->trajs.as<TrajDev>();
   p.layouts = E.layouts_dev.as<LayoutDev>();
@@ -1031,9 +1116,7 @@ int enqueue_fp64(Batch* B, const kmc_run_opts& o, hipStream_t s, const double* u
   p.traffic = traffic;
   p.occ_time = rec ? B->occ_time.as<double>() : nullptr;
   p.se_out = se_out;
-  void* args[] = {&p};
-  HIPOK(hipLaunchKernel((const void*)k, dim3((unsigned)B->ntraj), dim3(64), args, lds, s));
-  return KMC_OK;
+  return launch_plain(p, B->smax, B->nmax, rec, B->ntraj, s);
 }
 
 int batch_run_locked(int32_t id, const kmc_run_opts* opts) {
@@ -1558,6 +1641,132 @@ int kmc_simulate_discrete(int64_t N, int64_t P, double nu, double kT, double I_0
   cleanup(KMC_OK);
   if (status != KMC_OK)
     return fail(status, "total rate became zero after %lld hops (reference: ZeroDivisionError)",
+                (long long)r.hops_done);
+  return KMC_OK;
+}
+
+// kmc_simulate_discrete without allocation: the same kernel and results, with
+// every device buffer in the `scratch` array (its address from set-up) and the
+// host side working on the caller's arrays and stack records only.
+int kmc_simulate_discrete_noalloc(int64_t N, int64_t P, double nu, double kT, double I_0,
+                                  double R, const double* distances, const double* E_constant,
+                                  double* site_energies, const double* transitions_constant,
+                                  uint8_t* occupation, int64_t* electrode_occupation,
+                                  int64_t hops, int record, double* traffic,
+                                  double* occupations_in_time, const double* uniforms,
+                                  uint64_t seed, uint64_t hop0, int32_t* events_out,
+                                  double* totals_out, double* time_out) {
+  Engine& E = eng();
+  std::lock_guard<std::mutex> lk(E.mu);
+  int st = ensure_init();
+  if (st) return st;
+  char* base = static_cast<char*>(noalloc_scratch[E.device]);
+  if (!base) return fail(KMC_ERR_DEVICE, "scratch array not set up on device %d", E.device);
+  if (!site_energies || !occupation || (P > 0 && !electrode_occupation) || !E_constant)
+    return fail(KMC_ERR_ARGS, "null argument");
+  if ((st = check_network(N, P, KMC_NOALLOC_MAX_SITES, nu, kT, I_0, R, distances,
+                          transitions_constant)))
+    return st;
+  if (hops < 0 || hops > INT32_MAX) return fail(KMC_ERR_ARGS, "hops out of range");
+  if ((uniforms || events_out || totals_out) && hops > KMC_NOALLOC_MAX_LOGGED_HOPS)
+    return fail(KMC_ERR_LIMIT, "replayed uniforms or event logs hold at most %d hops here (got %lld)",
+                KMC_NOALLOC_MAX_LOGGED_HOPS, (long long)hops);
+  for (int64_t i = 0; i < N; ++i)
+    if (!std::isfinite(E_constant[i])) return fail(KMC_ERR_ARGS, "E_constant[%lld] is not finite", (long long)i);
+  for (int64_t e = 0; e < P; ++e)
+    if (!std::isfinite(site_energies[N + e]))
+      return fail(KMC_ERR_ARGS, "V[%lld] is not finite", (long long)e);
+  const int S = (int)(N + P), n = (int)N;
+  const size_t SS = (size_t)S * S;
+  hipStream_t s = E.stream;
+  LayoutDev* dL = reinterpret_cast<LayoutDev*>(base + NA_DESC);
+  EsetDev* dE = reinterpret_cast<EsetDev*>(base + NA_DESC + 256);
+  TrajDev* dT = reinterpret_cast<TrajDev*>(base + NA_DESC + 512);
+  TrajOut* dO = reinterpret_cast<TrajOut*>(base + NA_DESC + 640);
+  double* dEc = reinterpret_cast<double*>(base + NA_E);
+  double* dV = reinterpret_cast<double*>(base + NA_V);
+  double* dd = reinterpret_cast<double*>(base + NA_D);
+  double* dtc = reinterpret_cast<double*>(base + NA_TC);
+  double* dird = reinterpret_cast<double*>(base + NA_IRD);
+  double* dinvd = reinterpret_cast<double*>(base + NA_INVD);
+  double* dtr = reinterpret_cast<double*>(base + NA_TRAFFIC);
+  double* dse = reinterpret_cast<double*>(base + NA_SE);
+  double* dot = reinterpret_cast<double*>(base + NA_OT);
+  double* du = reinterpret_cast<double*>(base + NA_U);
+  int32_t* dev = reinterpret_cast<int32_t*>(base + NA_EV);
+  double* dtot = reinterpret_cast<double*>(base + NA_TOT);
+  static_assert(sizeof(LayoutDev) <= 256 && sizeof(EsetDev) <= 256 && sizeof(TrajDev) <= 128 &&
+                    640 + sizeof(TrajOut) <= 1024,
+                "descriptor slots");
+  LayoutDev Ld{};
+  Ld.N = n;
+  Ld.P = (int)P;
+  Ld.S = S;
+  Ld.nu = nu;
+  Ld.kT = kT;
+  Ld.I0 = I_0;
+  Ld.R = R;
+  Ld.I0R = I_0 * R;
+  Ld.tc64 = dtc;
+  Ld.ird64 = dird;
+  Ld.invd64 = dinvd;
+  Ld.dist64 = dd;
+  const EsetDev Ed{dEc, dV};
+  TrajDev Td{};
+  Td.layout = 0;
+  Td.eset = 0;
+  Td.seed = seed;
+  Td.hop0 = hop0;
+  Td.hops = hops;
+  Td.h_end = hops;
+  for (int64_t i = 0; i < N; ++i)
+    if (occupation[i]) Td.occ[i >> 6] |= 1ull << (i & 63);
+  const bool rec = record != 0;
+  HIPOK(hipMemcpyAsync(dL, &Ld, sizeof Ld, hipMemcpyHostToDevice, s));
+  HIPOK(hipMemcpyAsync(dE, &Ed, sizeof Ed, hipMemcpyHostToDevice, s));
+  HIPOK(hipMemcpyAsync(dT, &Td, sizeof Td, hipMemcpyHostToDevice, s));
+  if (N) HIPOK(hipMemcpyAsync(dEc, E_constant, sizeof(double) * N, hipMemcpyHostToDevice, s));
+  if (P) HIPOK(hipMemcpyAsync(dV, site_energies + N, sizeof(double) * P, hipMemcpyHostToDevice, s));
+  HIPOK(hipMemcpyAsync(dd, distances, sizeof(double) * SS, hipMemcpyHostToDevice, s));
+  HIPOK(hipMemcpyAsync(dtc, transitions_constant, sizeof(double) * SS, hipMemcpyHostToDevice, s));
+  if (uniforms && hops)
+    HIPOK(hipMemcpyAsync(du, uniforms, sizeof(double) * 2 * hops, hipMemcpyHostToDevice, s));
+  {
+    double I0R = I_0 * R;
+    int clear = rec ? 1 : 0, nn = n, SSi = S;
+    void* a[] = {&dd, &dird, &dinvd, &dtr, &dot, &nn, &SSi, &I0R, &clear};
+    HIPOK(hipLaunchKernel((const void*)kmc_noalloc_prep_kernel, dim3((unsigned)((SS + 255) / 256)),
+                          dim3(256), a, 0, s));
+  }
+  PlainParams p{};
+  p.trajs = dT;
+  p.layouts = dL;
+  p.esets = dE;
+  p.out = dO;
+  p.uniforms = uniforms ? du : nullptr;
+  p.events = events_out ? dev : nullptr;
+  p.totals = totals_out ? dtot : nullptr;
+  p.traffic = (rec && traffic) ? dtr : nullptr;
+  p.occ_time = rec ? dot : nullptr;
+  p.se_out = dse;
+  if ((st = launch_plain(p, S, n, rec, 1, s))) return st;
+  TrajOut r;
+  HIPOK(hipMemcpyAsync(&r, dO, sizeof r, hipMemcpyDeviceToHost, s));
+  if (hops > 0 && N) HIPOK(hipMemcpyAsync(site_energies, dse, sizeof(double) * N, hipMemcpyDeviceToHost, s));
+  if (events_out && hops)
+    HIPOK(hipMemcpyAsync(events_out, dev, sizeof(int32_t) * hops, hipMemcpyDeviceToHost, s));
+  if (totals_out && hops)
+    HIPOK(hipMemcpyAsync(totals_out, dtot, sizeof(double) * hops, hipMemcpyDeviceToHost, s));
+  if (rec && traffic) HIPOK(hipMemcpyAsync(traffic, dtr, sizeof(double) * SS, hipMemcpyDeviceToHost, s));
+  if (rec && occupations_in_time && N)
+    HIPOK(hipMemcpyAsync(occupations_in_time, dot, sizeof(double) * N, hipMemcpyDeviceToHost, s));
+  hipError_t se = hipStreamSynchronize(s);
+  if (se != hipSuccess) return fail(KMC_ERR_DEVICE, "kernel: %s", hipGetErrorString(se));
+  for (int64_t i = 0; i < N; ++i) occupation[i] = (r.occ[i >> 6] >> (i & 63)) & 1ull;
+  for (int64_t e = 0; e < P; ++e) electrode_occupation[e] += r.eo[e];
+  if (time_out) *time_out = r.time;
+  if (r.status != KMC_OK)
+    return fail(r.status, "total rate became zero after %lld hops (reference: ZeroDivisionError)",
                 (long long)r.hops_done);
   return KMC_OK;
 }
