@@ -68,8 +68,10 @@ def host_cores() -> int:
     return max(1, n)
 
 
-def cpu_baselines(ens, seconds: float) -> tuple[dict, dict]:
-    """Oracle restatements timed on this host (numba loop FP64; Go loop FP32)."""
+def cpu_baselines(ens, seconds: float) -> tuple[dict, dict, dict]:
+    """Oracle restatements timed on this host: the numba loop (FP64), the Go loop
+    (FP32) without and with its occupation-state cache (simulation.go:222-296,
+    here with a collision-free key), and the cache's hit rate."""
     import oracle
     net = ens.net
     cores = host_cores()
@@ -97,7 +99,51 @@ def cpu_baselines(ens, seconds: float) -> tuple[dict, dict]:
           "sample": f"oracle restatement of the Go loop (FP32, incremental energies, no cache, "
                     f"simulation.go:194-325), {ntraj} x {hops} hops, {dt:.1f} s",
           "status_ok": bool((st == 0).all())}
-    return numba, go
+    # the same loop with the Go library's occupation-state cache (record_problist):
+    # a revisited occupation reuses its stored cumulative rate list
+    stats = np.zeros((ntraj, 3), np.int64)
+    chops = hops * 32
+    t0 = time.perf_counter()
+    _, _, _, st = oracle.go_ensemble(*args, occ0, seeds, chops, nthreads=cores, use_cache=True,
+                                     cache_max_bytes=1 << 26, cache_stats=stats)
+    dt = time.perf_counter() - t0
+    hit = float(stats[:, 0].sum() / max(1, stats[:, 0].sum() + stats[:, 1].sum()))
+    go_cache = {"value": ntraj * chops / dt, "unit": "hops/s", "cores": cores, "kind": "port",
+                "sample": f"oracle restatement of the Go loop with the occupation cache "
+                          f"(simulation.go:222-296, collision-free key), {ntraj} x {chops} hops "
+                          f"from an empty network, {dt:.1f} s",
+                "cache_hit_rate": hit,
+                "states_per_trajectory": float(stats[:, 2].mean()),
+                "status_ok": bool((st == 0).all())}
+    return numba, go, go_cache
+
+
+def fixture_cache_hit_rate(hops: int = 1_000_000, k: int = 4) -> dict:
+    """Hit rate of the Go cache on the reference's own +-300 kT fixture networks
+    (thesis_indrek/tests, the setting of its published cache timings,
+    printTables.py:118-119), go_simulation protocol: empty start."""
+    import oracle
+    fx = np.load(os.path.join(ROOT, "tests", "golden", "kmc_fixtures.npz"))
+    from kmc_dn_amd import tables
+    out = {}
+    for fam in ("rnd", "xor"):
+        stats = np.zeros((k, 3), np.int64)
+        N, P = int(fx["N"]), int(fx["P"])
+        acc, el = fx[f"{fam}_acceptors"][0], fx[f"{fam}_electrodes"][0]
+        d = tables.calc_distances(acc, el)
+        tc = tables.calc_transitions_constant(d, float(fx["ab"]), float(fx["nu"]))
+        se = np.zeros(N + P)
+        se[N:] = el[:, 3]
+        t0 = time.perf_counter()
+        oracle.go_ensemble(N, P, 1.0, 1.0, float(fx["I_0"]), float(fx["R"]), d,
+                           fx[f"{fam}_E_constant"][0], se, tc, np.zeros(N, np.uint8),
+                           np.arange(k, dtype=np.uint64), hops, use_cache=True,
+                           cache_max_bytes=1 << 26, cache_stats=stats, nthreads=k)
+        dt = time.perf_counter() - t0
+        out[fam] = {"hit_rate": float(stats[:, 0].sum() / (stats[:, 0].sum() + stats[:, 1].sum())),
+                    "states_per_trajectory": float(stats[:, 2].mean()),
+                    "hops_per_s_per_core": hops / dt}
+    return out
 
 
 def iv_leg(lay, args) -> dict:
@@ -157,9 +203,37 @@ def uneven_leg(args) -> dict:
                         "one trajectory per network; bucket kernels forked in one hipGraph"}
 
 
+def iv_job_leg(srv, nets, args) -> dict:
+    """Served IV jobs: Poisson arrivals of IV-sweep requests (16 voltages of
+    electrode 0 over [-2, 2], the others held at U(-1.5, 1.5), 4 replicas of 2e5
+    hops per voltage) over the 64 resident layouts, admitted by device memory and
+    run by continuous batching beside one another."""
+    from kmc_dn_amd import serving, workloads
+    out = {"workload": f"IV jobs on 64 resident 2D layouts N~U{{20..100}}, 8 electrodes: "
+                       f"electrode 0 over linspace(-2,2,{args.ivjob_points}), the others "
+                       f"U(-1.5,1.5); {args.ivjob_replicas} replicas x {args.ivjob_hops} hops "
+                       "per voltage; Poisson arrivals; latency = completion - arrival"}
+    for rate in args.ivjob_rates:
+        reqs = workloads.iv_request_stream(rate, max(2, int(rate * args.ivjob_seconds)), nets,
+                                           seed=100 + int(rate * 10), points=args.ivjob_points,
+                                           replicas=args.ivjob_replicas, hops=args.ivjob_hops)
+        sreq = [serving.ServeRequest(r.rid, r.t_arrive, r.layout, r.V, r.hops, r.replicas)
+                for r in reqs]
+        hops = int(sum(r.hops * r.V.shape[0] * r.replicas for r in sreq))
+        r0 = srv.rounds
+        done, wall = serving.serve_stream(srv, sreq, realtime=True, max_wall=300)
+        d = serving.latency_summary(done, wall, hops)
+        d.update(offered_req_per_s=rate, requests=len(sreq), rounds=srv.rounds - r0,
+                 offered_hops_per_s=hops / max(sreq[-1].t_arrive, 1e-9))
+        assert d["failed"] == 0
+        out[f"{rate}/s"] = d
+    return out
+
+
 def serving_leg(args) -> dict:
     """Poisson request streams over 64 resident layouts, served by continuous
-    batching; beside them the one-launch-per-request baseline."""
+    batching; beside them the one-launch-per-request baseline, and the served
+    IV-job streams on the same server."""
     from kmc_dn_amd import serving, workloads
     nets, bases = workloads.resident_layouts(seed=0, n=64, N_range=(20, 100))
     srv = serving.Server(nets, bases, chunk_hops=args.serve_chunk)
@@ -193,6 +267,8 @@ def serving_leg(args) -> dict:
                  sample=f"first {len(sreq)} requests of the {args.serve_rates[0]}/s stream, "
                         "each launched alone on one stream as it arrives")
         out["one_launch_per_request"] = d
+        if not args.no_ivjobs:
+            out["iv_jobs"] = iv_job_leg(srv, nets, args)
     finally:
         srv.close()
     return out
@@ -222,6 +298,13 @@ def main() -> int:
     ap.add_argument("--serve-chunk", type=int, default=8192)
     ap.add_argument("--serve-baseline-n", type=int, default=48)
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--ivjob-rates", type=float, nargs="+", default=[2.0, 5.0, 10.0])
+    ap.add_argument("--ivjob-seconds", type=float, default=4.0)
+    ap.add_argument("--ivjob-points", type=int, default=16)
+    ap.add_argument("--ivjob-replicas", type=int, default=4)
+    ap.add_argument("--ivjob-hops", type=int, default=200_000)
+    ap.add_argument("--no-ivjobs", action="store_true")
+    ap.add_argument("--no-fp64-ensemble", action="store_true")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -276,6 +359,38 @@ def main() -> int:
     pwall = time.perf_counter() - tp
     plain_value = args.plain_hops * args.plain_launches / pwall
 
+    # the same FP64 kernel over the whole ensemble (one wave per trajectory): the
+    # reference arithmetic at full occupancy, and the FP32 currents' error against it
+    fp64_ens = None
+    if not args.no_fp64_ensemble:
+        b64 = engine.Batch(engine.make_trajs(args.ntraj, lay.id, es[0], ens.seeds, args.hops,
+                                             net.occupation))
+        t64 = time.perf_counter()
+        b64.run(precision=_lib.FP64_DENSE)
+        ms64 = b64.sync()
+        w64 = time.perf_counter() - t64
+        r64 = b64.results()
+        b64.close()
+        assert (r64.status == 0).all()
+        c32, c64 = cur, r64.currents(net.P)
+        m32, m64 = c32.mean(0), c64.mean(0)
+        se = np.sqrt(c32.var(0, ddof=1) / len(c32) + c64.var(0, ddof=1) / len(c64))
+        diff = np.abs(m32 - m64)
+        z = np.where(se > 0, diff / np.where(se > 0, se, 1), np.where(diff > 0, np.inf, 0))
+        resolvable = np.abs(m64) > 10 * se
+        rel = diff / np.maximum(np.abs(m64), 1e-300)
+        fp64_ens = {"value": hops_per_step / w64, "unit": "hops/s", "kernel_ms": ms64,
+                    "sample": f"{args.ntraj} trajectories x {args.hops} hops, FP64 dense kernel, "
+                              "one wave per trajectory, one launch"}
+        current_error = {
+            "vs": "FP64 dense path (the numba loop's arithmetic) on the same Philox streams",
+            "max_abs_z": float(z.max()),
+            "max_rel_resolvable": float(rel[resolvable].max()) if resolvable.any() else None,
+            "resolvable": int(resolvable.sum()), "unresolvable": int((~resolvable).sum()),
+            "resolvable_rule": "|I_fp64| > 10 standard errors of the difference",
+            "per_electrode_rel": [float(x) for x in rel],
+            "per_electrode_z": [float(x) for x in z]}
+
     out = {
         "metric": METRIC,
         "value": value,
@@ -324,6 +439,10 @@ def main() -> int:
         "speedup_vs_plain_fp64": value / plain_value,
         "mean_current": [float(x) for x in cur.mean(0)],
     }
+    if fp64_ens is not None:
+        out["plain_fp64_ensemble"] = fp64_ens
+        out["speedup_vs_fp64_ensemble"] = value / fp64_ens["value"]
+        out["current_error"] = current_error
     if not args.no_iv:
         out["iv_sweep"] = iv_leg(lay, args)
     if not args.no_uneven:
@@ -333,10 +452,15 @@ def main() -> int:
         out["latency_ms_p50_p99"] = {k: [v.get("p50_ms"), v.get("p99_ms")]
                                      for k, v in out["serving"].items()
                                      if isinstance(v, dict) and "p50_ms" in v}
+        for k, v in out["serving"].get("iv_jobs", {}).items():
+            if isinstance(v, dict) and "p50_ms" in v:
+                out["latency_ms_p50_p99"][f"iv_jobs {k}"] = [v["p50_ms"], v["p99_ms"]]
     if not args.no_cpu_baseline:
-        numba, go = cpu_baselines(ens, args.cpu_seconds)
+        numba, go, go_cache = cpu_baselines(ens, args.cpu_seconds)
         out["cpu_baseline"] = numba
         out["cpu_baseline_go"] = go
+        out["cpu_baseline_go_cache"] = go_cache
+        out["go_cache_hit_rate_fixtures_300kT"] = fixture_cache_hit_rate()
     print(json.dumps(out), flush=True)
     return 0
 

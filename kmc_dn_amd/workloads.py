@@ -206,3 +206,33 @@ def request_stream(rate: float, n_req: int, layouts: list, seed: int = 1,
     V = rng.uniform(-vmax, vmax, (n_req, 8))
     hops = np.minimum(np.exp(np.log(median_hops) + sigma * rng.randn(n_req)), max_hops)
     return [Request(i, float(t[i]), int(lay[i]), V[i], max(1, int(hops[i]))) for i in range(n_req)]
+
+
+@dataclass
+class IVRequest:
+    rid: int
+    t_arrive: float
+    layout: int
+    V: np.ndarray                 # (points, P): electrode `electrode` swept, the others held
+    hops: int
+    replicas: int
+
+
+def iv_request_stream(rate: float, n_req: int, layouts: list, seed: int = 1, points: int = 16,
+                      replicas: int = 4, hops: int = 200_000, electrode: int = 0,
+                      vrange=(-2.0, 2.0), vmax: float = 1.5) -> list:
+    """Poisson arrivals of IV-sweep requests (the evaluation of IV,
+    kmc_dopant_networks_utils.py:639-693, as a served job): each names a resident
+    layout, holds its other electrodes at U(-vmax, vmax) and sweeps `electrode`
+    over linspace(vrange, points), `replicas` trajectories of `hops` hops per point."""
+    rng = np.random.RandomState(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, n_req))
+    lay = rng.randint(0, len(layouts), n_req)
+    sweep = np.linspace(vrange[0], vrange[1], points)
+    out = []
+    for i in range(n_req):
+        P = layouts[int(lay[i])].P
+        V = np.repeat(rng.uniform(-vmax, vmax, P)[None], points, 0)
+        V[:, electrode] = sweep
+        out.append(IVRequest(i, float(t[i]), int(lay[i]), V, int(hops), int(replicas)))
+    return out
