@@ -336,6 +336,18 @@ int kmc_probe_copy(int64_t bytes, int repeats, double* ms_per_copy);
  * iteration (the LDS peak figure). */
 int kmc_probe_lds(int width_bytes, int blocks, int iters, double* bytes_per_s);
 
+/* Dependent-instruction latency of one wave in shader cycles per instruction:
+ * kind 0 v_add_f64 (all lanes), 1 v_add_f64 (one lane), 2 v_add_f32, 3
+ * v_fma_f64, 4 four interleaved independent v_add_f64 chains (issue rate). */
+int kmc_probe_chain(int kind, int iters, double* cycles_per_instruction);
+
+/* Runs the batch once on the FP64 dense kernel's phase-timing instantiation
+ * (shader-clock reads at the phase boundaries of every hop, same work) and
+ * returns trajectory 0's shader cycles summed per phase: cycles[0..6] =
+ * energies, pair lists, rates, prefix chain, uniforms and clock, event search,
+ * hop and records (n >= 7). */
+int kmc_probe_fp64_phases(int32_t batch, uint64_t* cycles, int32_t n);
+
 #ifdef __cplusplus
 }
 #endif

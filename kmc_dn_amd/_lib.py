@@ -96,6 +96,8 @@ SIGNATURES = {
     "kmc_batch_destroy": (C.c_int, [_i32]),
     "kmc_probe_copy": (C.c_int, [_i64, C.c_int, _dp]),
     "kmc_probe_lds": (C.c_int, [C.c_int, C.c_int, C.c_int, _dp]),
+    "kmc_probe_fp64_phases": (C.c_int, [_i32, _u64p, _i32]),
+    "kmc_probe_chain": (C.c_int, [C.c_int, C.c_int, _dp]),
 }
 
 _lib = None

@@ -20,7 +20,7 @@
 
 namespace kmc {
 
-template <bool REC, bool STAGE>
+template <bool REC, bool STAGE, bool PHASE>
 __global__ void kmc_plain_fp64_kernel(PlainParams);
 template <int C, int W, bool REC, bool BT, bool PRIO>
 __global__ void kmc_batch_fp32_kernel(BatchParams);
@@ -1087,8 +1087,13 @@ int launch_plain(PlainParams p, int S, int N, bool rec, int64_t ntraj, hipStream
   const bool stage = plain_lds_bytes(S, N, true) <= cap;
   const size_t lds = plain_lds_bytes(S, N, stage);
   if (lds > cap) return fail(KMC_ERR_LIMIT, "FP64 kernel needs %zu bytes of LDS", lds);
-  void (*k)(PlainParams) = rec ? (stage ? kmc_plain_fp64_kernel<true, true> : kmc_plain_fp64_kernel<true, false>)
-                               : (stage ? kmc_plain_fp64_kernel<false, true> : kmc_plain_fp64_kernel<false, false>);
+  using K = void (*)(PlainParams);
+  K k = rec ? (stage ? (K)kmc_plain_fp64_kernel<true, true, false> : (K)kmc_plain_fp64_kernel<true, false, false>)
+            : (stage ? (K)kmc_plain_fp64_kernel<false, true, false> : (K)kmc_plain_fp64_kernel<false, false, false>);
+  if (p.phases) {
+    if (rec) return fail(KMC_ERR_UNSUPPORTED, "phase timing runs without records");
+    k = stage ? (K)kmc_plain_fp64_kernel<false, true, true> : (K)kmc_plain_fp64_kernel<false, false, true>;
+  }
   HIPOK(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   void* args[] = {&p};
   HIPOK(hipLaunchKernel((const void*)k, dim3((unsigned)ntraj), dim3(64), args, lds, s));
@@ -1096,7 +1101,8 @@ int launch_plain(PlainParams p, int S, int N, bool rec, int64_t ntraj, hipStream
 }
 
 int enqueue_fp64(Batch* B, const kmc_run_opts& o, hipStream_t s, const double* uniforms,
-                 int32_t* events, double* totals, double* traffic, double* se_out) {
+                 int32_t* events, double* totals, double* traffic, double* se_out,
+                 uint64_t* phases = nullptr) {
   Engine& E = eng();
   if (B->smax > SMAX_FP64)
     return fail(KMC_ERR_LIMIT, "FP64 dense kernel supports S <= %d (got %d)", SMAX_FP64, B->smax);
@@ -1116,6 +1122,7 @@ int enqueue_fp64(Batch* B, const kmc_run_opts& o, hipStream_t s, const double* u
   p.traffic = traffic;
   p.occ_time = rec ? B->occ_time.as<double>() : nullptr;
   p.se_out = se_out;
+  p.phases = phases;
   return launch_plain(p, B->smax, B->nmax, rec, B->ntraj, s);
 }
 
@@ -1435,6 +1442,29 @@ int kmc_batch_traffic(int32_t batch, double* traffic, int64_t* stride_out) {
   for (int64_t i = 0; i < B->ntraj; ++i)
     memcpy(traffic + (size_t)B->perm[i] * stride, t.data() + (size_t)i * stride,
            sizeof(double) * stride);
+  return KMC_OK;
+}
+
+int kmc_probe_fp64_phases(int32_t batch, uint64_t* cycles, int32_t n) {
+  Engine& E = eng();
+  std::lock_guard<std::mutex> lk(E.mu);
+  int st = ensure_init();
+  if (st) return st;
+  Batch* B = get_batch(batch);
+  if (!B) return fail(KMC_ERR_ARGS, "bad batch id %d", batch);
+  if (!cycles || n < PLAIN_PHASES) return fail(KMC_ERR_ARGS, "need %d phase slots", PLAIN_PHASES);
+  if ((st = sync_registry())) return st;
+  DevBuf ph;
+  if ((st = ph.alloc(sizeof(uint64_t) * PLAIN_PHASES))) return st;
+  HIPOK(hipMemset(ph.p, 0, sizeof(uint64_t) * PLAIN_PHASES));
+  kmc_run_opts o{};
+  o.precision = KMC_FP64_DENSE;
+  if ((st = enqueue_fp64(B, o, E.stream, nullptr, nullptr, nullptr, nullptr, nullptr,
+                         ph.as<uint64_t>())))
+    return st;
+  HIPOK(hipStreamSynchronize(E.stream));
+  HIPOK(hipMemcpy(cycles, ph.p, sizeof(uint64_t) * PLAIN_PHASES, hipMemcpyDeviceToHost));
+  B->last_rec = B->last_traffic = false;
   return KMC_OK;
 }
 

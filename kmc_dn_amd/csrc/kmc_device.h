@@ -161,6 +161,20 @@ __device__ __forceinline__ float wave_scan_f32(float x) {
   return x;
 }
 
+// Inclusive max-scan of unsigned 32-bit values over the 64 lanes (the DPP row
+// shifts and row broadcasts of wave_scan_f64, with 0 as the identity).  Every
+// lane must be active.
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t x) {
+  auto step = [](uint32_t v, uint32_t s) { return v > s ? v : s; };
+  x = step(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, DPP_ROW_SHR1, 0xF, 0xF, true));
+  x = step(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, DPP_ROW_SHR2, 0xF, 0xF, true));
+  x = step(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, DPP_ROW_SHR4, 0xF, 0xF, true));
+  x = step(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, DPP_ROW_SHR8, 0xF, 0xF, true));
+  x = step(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, DPP_ROW_BCAST15, 0xA, 0xF, false));
+  x = step(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, DPP_ROW_BCAST31, 0xC, 0xF, false));
+  return x;
+}
+
 __device__ __forceinline__ float fast_exp2(float x) { return __builtin_amdgcn_exp2f(x); }
 __device__ __forceinline__ float fast_log2(float x) { return __builtin_amdgcn_logf(x); }
 

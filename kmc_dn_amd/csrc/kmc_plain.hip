@@ -23,8 +23,10 @@
 // (occupied acceptors; empty acceptors followed by electrodes) built with
 // ballots give every list position its (row, column) by one small division,
 // so the rates are lane-parallel over the compacted list.  Only the prefix
-// chain is serial: one lane runs it from registers, 16 values per LDS block
-// with the next block's loads issued before the current block's adds.  The
+// chain is serial: one lane runs it from registers, 8 values per LDS block in
+// two alternating register sets (the next block's loads in flight during the
+// current block's adds).  Uniforms and the clock's -log(1 - u) are computed
+// lane-parallel for 64 hops at a time.  The
 // event search is monotone (the prefix never decreases and the quotient by
 // the total is monotone), so two lane-parallel rounds find the first index
 // (a coarse pass over 64 block ends, then the block).  With STAGE the layout's
@@ -43,8 +45,8 @@ struct PlainLds {
     buf = up2(S);
     // possible pairs: nA (N - nA + P) + P (N - nA) <= N^2/4 + P N for nA occupied
     const size_t npairs = (size_t)N * N / 4 + (size_t)(S - N) * N + 16;
-    lists = buf + up2(npairs);                  // int16: occupied acceptors [N], targets [S]
-    invd = lists + up2(((size_t)N + S + 3) / 4);
+    lists = buf + up2(npairs);   // int16: occupied acceptors [N], targets [S], kept [npairs]
+    invd = lists + up2(((size_t)N + S + npairs + 3) / 4);
     tc = invd + (stage ? up2((size_t)N * N) : 0);
     ird = tc + (stage ? up2(SS) : 0);
     total = ird + (stage ? up2(SS) : 0);
@@ -67,7 +69,9 @@ __device__ __forceinline__ int div_small(int x, int d, float rcp, int& r) {
   return q;
 }
 
-template <bool REC, bool STAGE>
+// PHASE: the phase-timing instantiation (shader-clock reads at the phase
+// boundaries of every hop; the work is unchanged).
+template <bool REC, bool STAGE, bool PHASE>
 __global__ void __launch_bounds__(64) kmc_plain_fp64_kernel(PlainParams p) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int t = blockIdx.x;
@@ -88,6 +92,7 @@ __global__ void __launch_bounds__(64) kmc_plain_fp64_kernel(PlainParams p) {
   double* buf = dl + off.buf;     // rates of the possible pairs, then their prefix sums
   int16_t* srcA = reinterpret_cast<int16_t*>(dl + off.lists);   // occupied acceptors
   int16_t* tgt = srcA + N;        // empty acceptors, then electrodes N..S-1
+  int16_t* kidx = tgt + S;        // list positions of the kept rates
   const double* invd = L.invd64;
   const double* tcp = L.tc64;
   const double* irdp = L.ird64;
@@ -121,6 +126,38 @@ __global__ void __launch_bounds__(64) kmc_plain_fp64_kernel(PlainParams p) {
   int32_t status = ST_OK;
   const int64_t total_hops = td.prehops + td.hops;
   int64_t h = 0;
+  uint64_t ph[PLAIN_PHASES] = {};
+  uint64_t tp = PHASE ? __builtin_amdgcn_s_memtime() : 0;
+  // cycles since the last mark into phase k
+  auto mark = [&](int k) {
+    if constexpr (PHASE) {
+      const uint64_t now = __builtin_amdgcn_s_memtime();
+      ph[k] += now - tp;
+      tp = now;
+    }
+  };
+  // -log(1 - u_clock) and u_event of hop (h & ~63) + lane: 64 hops of uniforms
+  // and clock logarithms at a time, read with readlane per hop (the same
+  // values and operations as per hop)
+  double lg_lane = 0.0, ue_lane = 0.0;
+  auto refill = [&](int64_t hh) {
+    const int64_t hk = (hh & ~(int64_t)63) + lane;
+    double uc = 0.5, ue = 0.5;
+    if (p.uniforms) {
+      if (hk < total_hops) {
+        uc = p.uniforms[2 * hk];
+        ue = p.uniforms[2 * hk + 1];
+      }
+    } else {
+      stream_uniforms(td.seed, td.hop0 + (uint64_t)hk, uc, ue);
+    }
+    lg_lane = -log(1.0 - uc);
+    ue_lane = ue;
+  };
+  // every scalar load of the set-up has returned before the loop: LDS
+  // operations alone then complete in order, so the waits for the prefix
+  // chain's blocks count only the operations issued after each block
+  __builtin_amdgcn_s_waitcnt(0x0070);   // vmcnt(0) lgkmcnt(0)
   for (; h < total_hops; ++h) {
     const bool tally = h >= td.prehops;
     if (h == td.prehops) {   // reset() between prehops and hops (:562-585)
@@ -128,26 +165,10 @@ __global__ void __launch_bounds__(64) kmc_plain_fp64_kernel(PlainParams p) {
       eo = 0;
       ot0 = ot1 = 0.0;
     }
+    if ((h & 63) == 0) refill(h);
     // occupied / empty acceptors as wave-uniform masks of the two site chunks
     const uint64_t o0 = occ.a & acc0, o1 = occ.b & acc1;
     const uint64_t e0 = acc0 & ~o0, e1 = acc1 & ~o1;
-    // site energies, :59-66: the empty acceptors j in ascending order
-    {
-      double a0 = 0.0, a1 = 0.0;
-#pragma unroll
-      for (int w = 0; w < 2; ++w) {
-        uint64_t m = w ? e1 : e0;
-        while (m) {
-          const int b = __builtin_ctzll(m);
-          m &= m - 1;
-          const int j = b + 64 * w;
-          if (lane < N) a0 += invd[j * N + lane];
-          if (lane + 64 < N) a1 += invd[j * N + lane + 64];
-        }
-      }
-      if (lane < N) se[lane] = E0 + nIR * a0;
-      if (lane + 64 < N) se[lane + 64] = E1 + nIR * a1;
-    }
     // the lists: occupied acceptors, then targets (empty acceptors, electrodes)
     const int nA = __builtin_popcountll(o0) + __builtin_popcountll(o1);
     const int cE = N - nA;
@@ -165,118 +186,179 @@ __global__ void __launch_bounds__(64) kmc_plain_fp64_kernel(PlainParams p) {
     const int nAA = nA * cA;
     const int n = nAA + P * cE;
     wave_lds_sync();
-    // rates of the possible pairs in row-major order, :69-87
-    const float rA = cA > 0 ? __builtin_amdgcn_rcpf((float)cA) : 0.f;
-    const float rE = cE > 0 ? __builtin_amdgcn_rcpf((float)cE) : 0.f;
-    for (int b0 = 0; b0 < n; b0 += 64) {
-      const int idx = b0 + lane;
-      if (idx < n) {
-        int i, j, r;
-        if (idx < nAA) {
-          const int q = div_small(idx, cA, rA, r);
-          i = srcA[q];
-          j = tgt[r];
-        } else {
-          const int q = div_small(idx - nAA, cE, rE, r);
-          i = N + q;
-          j = tgt[r];
+    mark(1);
+    // site energies, :59-66: the empty acceptors j (tgt[0..cE), ascending),
+    // four at a time with their loads issued together, added in order
+    {
+      double a0 = 0.0, a1 = 0.0;
+      const int l0 = min(lane, N - 1), l1 = min(lane + 64, N - 1);
+      for (int q0 = 0; q0 < cE; q0 += 4) {
+        const int cnt = min(4, cE - q0);
+        int j[4];
+        double x0[4], x1[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) j[q] = tgt[q0 + min(q, cnt - 1)];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) x0[q] = invd[j[q] * N + l0];
+        if (N > 64) {
+#pragma unroll
+          for (int q = 0; q < 4; ++q) x1[q] = invd[j[q] * N + l1];
         }
-        const int k = i * S + j;
-        double dE;
-        if (i < N && j < N) dE = se[j] - se[i] - irdp[k];
-        else dE = se[j] - se[i];
-        double m;
-        if (dE > 0) {
-          const double x = kT1 ? -dE : -dE / kT;
-          const double ex = exp(x);
-          m = nu1 ? ex : nu * ex;
-        } else {
-          m = nu;
-        }
-        buf[idx] = tcp[k] * m;
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          if (q < cnt) {
+            a0 += x0[q];
+            if (N > 64) a1 += x1[q];
+          }
       }
+      if (lane < N) se[lane] = E0 + nIR * a0;
+      if (lane + 64 < N) se[lane + 64] = E1 + nIR * a1;
     }
     wave_lds_sync();
-    // cumulative sum, :91-97: one lane, 16 values per block, the next block's
-    // loads in flight during this block's adds
+    mark(0);
+    // rates of the possible pairs in row-major order, :69-87; two list entries
+    // per lane and round, computed side by side (entries past n are evaluated
+    // at n - 1 and dropped).
+    //
+    // Absorbed entries are dropped too.  All rates are >= 0, so the running
+    // sum s_{k-1} before entry k is at least the largest earlier rate M, and a
+    // rate below half an ulp of M (so below half an ulp of s_{k-1}) leaves the
+    // rounded sum unchanged: fl(s + r) == s.  M is bounded from below by the
+    // running maximum of the rates' high words (their low 32 bits cleared),
+    // a wave max-scan on 32-bit integers; every partial sum the prefix chain
+    // forms over the kept entries is the reference's partial sum at that
+    // entry, and the dropped entries' partial sums equal the kept entry's
+    // before them.  The kept rates go to buf[0..ns) and their list positions
+    // to kidx.
+    const float rA = cA > 0 ? __builtin_amdgcn_rcpf((float)cA) : 0.f;
+    const float rE = cE > 0 ? __builtin_amdgcn_rcpf((float)cE) : 0.f;
+    auto pair_of = [&](int idx, int& i, int& j) {
+      int r;
+      if (idx < nAA) {
+        const int q = div_small(idx, cA, rA, r);
+        i = srcA[q];
+      } else {
+        const int q = div_small(idx - nAA, cE, rE, r);
+        i = N + q;
+      }
+      j = tgt[r];
+    };
+    auto rate_of = [&](int i, int j) {
+      const int k = i * S + j;
+      double dE;
+      if (i < N && j < N) dE = se[j] - se[i] - irdp[k];
+      else dE = se[j] - se[i];
+      const double x = kT1 ? -dE : -dE / kT;
+      const double ex = exp(x);
+      const double m = dE > 0 ? (nu1 ? ex : nu * ex) : nu;
+      return tcp[k] * m;
+    };
+    uint32_t hcarry = 0;   // running maximum of the high words (wave-uniform)
+    int ns = 0;            // entries kept so far (wave-uniform)
+    auto keep = [&](double r, int idx) {
+      const bool valid = idx < n;
+      const uint32_t hi = valid ? (uint32_t)((uint64_t)__double_as_longlong(r) >> 32) : 0u;
+      const uint32_t m = max(wave_max_u32(hi), hcarry);
+      hcarry = (uint32_t)__builtin_amdgcn_readlane((int)m, 63);
+      const uint32_t e = (m >> 20) & 0x7FFu;   // biased exponent of the bound
+      const uint32_t thr = e > 53u ? (e - 53u) << 20 : 0u;   // high word of its half ulp
+      const bool kp = valid && hi >= thr;
+      const uint64_t km = ballot(kp);
+      if (kp) {
+        const int pos = ns + mbcnt(km);
+        buf[pos] = r;
+        kidx[pos] = (int16_t)idx;
+      }
+      ns += __builtin_popcountll(km);
+    };
+    for (int b0 = 0; b0 < n; b0 += 128) {
+      const int ia = b0 + lane, ib = b0 + 64 + lane;
+      int i0, j0, i1, j1;
+      pair_of(ia < n ? ia : n - 1, i0, j0);
+      pair_of(ib < n ? ib : n - 1, i1, j1);
+      const double r0 = rate_of(i0, j0);
+      const double r1 = rate_of(i1, j1);
+      keep(r0, ia);
+      keep(r1, ib);
+    }
+    wave_lds_sync();
+    mark(2);
+    // cumulative sum, :91-97, over the kept entries: one lane; blocks of 8 in
+    // two register sets that alternate, so the next block's loads are in
+    // flight during this block's adds
     double acc = 0.0;
     if (lane == 0) {
-      const double2* src = reinterpret_cast<const double2*>(buf);
-      double2* dst = reinterpret_cast<double2*>(buf);
-      const int nb = n >> 4;
-      double2 cur[8], nxt[8];
-      if (nb > 0) {
-#pragma unroll
-        for (int q = 0; q < 8; ++q) cur[q] = src[q];
-      }
-#pragma unroll 1
-      for (int blk = 0; blk < nb; ++blk) {
-        if (blk + 1 < nb) {
-#pragma unroll
-          for (int q = 0; q < 8; ++q) nxt[q] = src[(blk + 1) * 8 + q];
-        }
+      const int nb = ns >> 3;
+      double A[8], B[8];
+      // the running sum through one block of 8, written back as it goes: the
+      // block's registers are only read, so they are free once it is done
+      auto chain = [&](const double (&v)[8], int base) {
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
-          acc = cur[q].x + acc;
-          cur[q].x = acc;
-          acc = cur[q].y + acc;
-          cur[q].y = acc;
+          acc = v[q] + acc;
+          buf[base + q] = acc;
         }
+      };
+      if (nb > 0) {
 #pragma unroll
-        for (int q = 0; q < 8; ++q) dst[blk * 8 + q] = cur[q];
-#pragma unroll
-        for (int q = 0; q < 8; ++q) cur[q] = nxt[q];
+        for (int q = 0; q < 8; ++q) A[q] = buf[q];
       }
-      for (int idx = nb << 4; idx < n; ++idx) {
+      int blk = 0;
+#pragma unroll 1
+      for (; blk + 1 < nb; blk += 2) {
+        // straight-line body: the reload reads an existing block even past the
+        // last pair (its values are then unused)
+        const int nxt = min(blk + 2, nb - 1) * 8;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) B[q] = buf[(blk + 1) * 8 + q];
+        __builtin_amdgcn_sched_barrier(0);   // keep the loads ahead of the adds
+        chain(A, blk * 8);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) A[q] = buf[nxt + q];
+        __builtin_amdgcn_sched_barrier(0);
+        chain(B, (blk + 1) * 8);
+      }
+      if (blk < nb) chain(A, blk * 8);
+      for (int idx = nb << 3; idx < ns; ++idx) {
         acc = buf[idx] + acc;
         buf[idx] = acc;
       }
     }
     const double total = readlane_f64(acc, 0);
+    mark(3);
     if (p.totals && t == 0 && lane == 0) p.totals[h] = total;
     if (total == 0.0) {   // reference: ZeroDivisionError at 1/problist[-1]
       status = ST_ZERO_RATE;
       break;
     }
-    double uc, ue;
-    if (p.uniforms) {
-      uc = p.uniforms[2 * h];
-      ue = p.uniforms[2 * h + 1];
-    } else {
-      stream_uniforms(td.seed, td.hop0 + (uint64_t)h, uc, ue);
-    }
+    const double lg = readlane_f64(lg_lane, (int)(h & 63));
+    const double ue = readlane_f64(ue_lane, (int)(h & 63));
     const double scale = 1.0 / total;
-    const double hop_time = -log(1.0 - uc) * scale;
+    const double hop_time = lg * scale;
     wave_lds_sync();
-    // first k with problist[k]/total >= u, :103-110 (monotone in k)
-    int k_ev = 0;
+    mark(4);
+    // first k with problist[k]/total >= u, :103-110 (monotone in k): a kept
+    // entry, since a dropped one repeats the partial sum of the kept entry
+    // before it
+    int from = 0, to = 0;   // u == 0 and no match: the reference's event 0, (0, 0)
     if (ue != 0.0) {
-      const int stride = (n + 63) >> 6;
-      const int end = min(n, (lane + 1) * stride) - 1;
-      const bool hit = lane * stride < n && buf[end] / total >= ue;
+      const int stride = (ns + 63) >> 6;
+      const int end = min(ns, (lane + 1) * stride) - 1;
+      const bool hit = lane * stride < ns && buf[end] / total >= ue;
       const uint64_t mc = ballot(hit);
-      int found = -1;
       if (mc) {
         const int base = __builtin_ctzll(mc) * stride;
         const int pos = base + lane;
-        const uint64_t mf = ballot(lane < stride && pos < n && buf[pos] / total >= ue);
-        found = base + __builtin_ctzll(mf);   // the block's last entry is a hit
-      }
-      if (found >= 0) {
-        int r;
-        if (found < nAA) {
-          const int q = div_small(found, cA, rA, r);
-          k_ev = srcA[q] * S + tgt[r];
-        } else {
-          const int q = div_small(found - nAA, cE, rE, r);
-          k_ev = (N + q) * S + tgt[r];
-        }
+        const uint64_t mf = ballot(lane < stride && pos < ns && buf[pos] / total >= ue);
+        pair_of(kidx[base + __builtin_ctzll(mf)], from, to);   // the block's last entry hits
       }
       // the same on every lane: keep the event and the occupation it updates
       // on the scalar unit
-      k_ev = __builtin_amdgcn_readfirstlane(k_ev);
+      from = __builtin_amdgcn_readfirstlane(from);
+      to = __builtin_amdgcn_readfirstlane(to);
     }
-    const int from = k_ev / S, to = k_ev - (k_ev / S) * S;
+    const int k_ev = from * S + to;
+    mark(5);
     if (p.events && t == 0 && lane == 0) p.events[h] = k_ev;
     // perform hop, :116-123
     if (from < N) occ.assign(from, false);
@@ -291,7 +373,10 @@ __global__ void __launch_bounds__(64) kmc_plain_fp64_kernel(PlainParams p) {
     }
     time += hop_time;
     wave_lds_sync();
+    mark(6);
   }
+  if (PHASE && p.phases && t == 0 && lane == 0)
+    for (int k = 0; k < PLAIN_PHASES; ++k) p.phases[k] = ph[k];
   TrajOut* o = p.out + t;
   if (lane == 0) {
     o->time = time;
@@ -311,10 +396,12 @@ __global__ void __launch_bounds__(64) kmc_plain_fp64_kernel(PlainParams p) {
     for (int i = lane; i < N; i += 64) p.se_out[(size_t)t * NMAX + i] = se[i];
 }
 
-template __global__ void kmc_plain_fp64_kernel<false, false>(PlainParams);
-template __global__ void kmc_plain_fp64_kernel<true, false>(PlainParams);
-template __global__ void kmc_plain_fp64_kernel<false, true>(PlainParams);
-template __global__ void kmc_plain_fp64_kernel<true, true>(PlainParams);
+template __global__ void kmc_plain_fp64_kernel<false, false, false>(PlainParams);
+template __global__ void kmc_plain_fp64_kernel<true, false, false>(PlainParams);
+template __global__ void kmc_plain_fp64_kernel<false, true, false>(PlainParams);
+template __global__ void kmc_plain_fp64_kernel<true, true, false>(PlainParams);
+template __global__ void kmc_plain_fp64_kernel<false, true, true>(PlainParams);
+template __global__ void kmc_plain_fp64_kernel<false, false, true>(PlainParams);
 
 // LDS bytes the kernel needs for S sites, N acceptors.
 size_t plain_lds_bytes(int S, int N, bool stage) {

@@ -69,6 +69,41 @@ __global__ void __launch_bounds__(256) kmc_probe_lds_kernel(int iters, uint32_t*
   if (acc == 0x12345678u) out[blockIdx.x] = acc;   // keeps the loads live
 }
 
+// Dependent-instruction latency: one wave runs `iters` x 32 instructions that
+// each depend on the previous one, timed with the shader clock.  kind 0:
+// v_add_f64 on all lanes; 1: v_add_f64 on lane 0 alone; 2: v_add_f32 on all
+// lanes; 3: v_fma_f64 on all lanes; 4: four independent v_add_f64 chains
+// interleaved (issue rate, per instruction).
+#define KMC_X4(s) s s s s
+#define KMC_X32(s) KMC_X4(KMC_X4(s)) KMC_X4(KMC_X4(s))
+__global__ void __launch_bounds__(64) kmc_probe_chain_kernel(int kind, int iters, double* out) {
+  double a = 1.0 + threadIdx.x * 1e-3, b = 1e-9, c = 2.0, d = 3.0, e = 4.0;
+  float f = 1.0f + threadIdx.x, g = 1e-7f;
+  asm volatile("" : "+v"(a), "+v"(b), "+v"(c), "+v"(d), "+v"(e), "+v"(f), "+v"(g));
+  uint64_t t0 = 0, t1 = 0;
+  if (kind == 1 && threadIdx.x != 0) return;
+  t0 = __builtin_amdgcn_s_memtime();
+  for (int it = 0; it < iters; ++it) {
+    if (kind <= 1)
+      asm volatile(KMC_X32("v_add_f64 %0, %0, %1\n\t") : "+v"(a) : "v"(b));
+    else if (kind == 2)
+      asm volatile(KMC_X32("v_add_f32 %0, %0, %1\n\t") : "+v"(f) : "v"(g));
+    else if (kind == 3)
+      asm volatile(KMC_X32("v_fma_f64 %0, %0, %1, %1\n\t") : "+v"(a) : "v"(b));
+    else
+      asm volatile(KMC_X4(KMC_X4("v_add_f64 %0, %0, %4\n\tv_add_f64 %1, %1, %4\n\t"
+                                 "v_add_f64 %2, %2, %4\n\tv_add_f64 %3, %3, %4\n\t"))
+                   : "+v"(a), "+v"(c), "+v"(d), "+v"(e) : "v"(b));
+  }
+  t1 = __builtin_amdgcn_s_memtime();
+  if (threadIdx.x == 0) {
+    out[0] = (double)(t1 - t0) / ((double)iters * (kind == 4 ? 64 : 32));
+    out[1] = a + c + d + e + (double)f;
+  }
+}
+#undef KMC_X32
+#undef KMC_X4
+
 }  // namespace kmc
 
 using namespace kmc;
@@ -128,6 +163,19 @@ int kmc_probe_lds(int width_bytes, int blocks, int iters, double* bytes_per_s) {
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   (void)hipFree(out);
+  return st == hipSuccess ? KMC_OK : KMC_ERR_DEVICE;
+}
+
+int kmc_probe_chain(int kind, int iters, double* cycles_per_instruction) {
+  if (kind < 0 || kind > 4 || iters <= 0 || !cycles_per_instruction) return KMC_ERR_ARGS;
+  double* out = nullptr;
+  if (hipMalloc(&out, 2 * sizeof(double)) != hipSuccess) return KMC_ERR_NOMEM;
+  hipLaunchKernelGGL(kmc_probe_chain_kernel, dim3(1), dim3(64), 0, 0, kind, 16, out);
+  hipLaunchKernelGGL(kmc_probe_chain_kernel, dim3(1), dim3(64), 0, 0, kind, iters, out);
+  double h[2] = {0, 0};
+  hipError_t st = hipMemcpy(h, out, sizeof h, hipMemcpyDeviceToHost);
+  (void)hipFree(out);
+  *cycles_per_instruction = h[0];
   return st == hipSuccess ? KMC_OK : KMC_ERR_DEVICE;
 }
 

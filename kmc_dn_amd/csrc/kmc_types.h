@@ -135,7 +135,14 @@ struct PlainParams {
   double* traffic;          // [ntraj * S*S] gross traffic, or null
   double* occ_time;         // [ntraj * NMAX] post-hop occupation time, or null
   double* se_out;           // [ntraj * NMAX] final site_energies[0:N], or null
+  uint64_t* phases;         // [PLAIN_PHASES] shader cycles per hop phase of trajectory 0
+                            // (the phase-timing instantiation only), or null
 };
+
+// Hop phases of the FP64 dense kernel timed by its phase-timing instantiation:
+// energies, pair lists, rates, prefix chain, uniforms and clock, event search,
+// hop and records.
+constexpr int PLAIN_PHASES = 7;
 
 // Status codes written per trajectory (also returned by the C-ABI).
 enum : int32_t {

@@ -5,7 +5,9 @@ copy: a plain device copy of a known byte count -- the reference kernel for
 lds:  LDS read bandwidth of strided ds_read_b32/b64/b128 (--width 4/8/16) and of
       the ds_read_b128 stream with immediate offsets (--width 0, the peak figure);
 lone: the hop latency of one trajectory alone (the long tail of an uneven batch
-      or of a served request) on a network of N acceptors and P electrodes.
+      or of a served request) on a network of N acceptors and P electrodes;
+fp64phases: shader cycles per hop and phase of the FP64 dense kernel;
+chain: latency of dependent FP64 / FP32 adds in one wave.
 """
 from __future__ import annotations
 
@@ -55,9 +57,59 @@ def lone(N: int, P: int, form: int, team: int, hops: int = 20_000, seed: int = 1
             "us_per_hop": ms * 1e3 / hops}
 
 
+FP64_PHASES = ["energies", "pair_lists", "rates", "prefix_chain", "uniforms_clock", "search",
+               "hop_records"]
+
+
+def fp64_phases(hops: int = 20_000, seed: int = 0) -> dict:
+    """Shader cycles per hop and phase of the FP64 dense kernel, one trajectory
+    alone on the N=30 network of the ensemble workload (the one-trajectory
+    launch of the plain path), from its phase-timing instantiation; beside them
+    the HIP-event time per hop of the ordinary kernel."""
+    import numpy as np
+    from . import workloads
+    net = workloads.network_2d(seed)
+    lay = engine.Layout(net.N, net.P, net.nu, net.kT, net.I_0, net.R, net.distances,
+                        net.transitions_constant)
+    try:
+        es = lay.energy_sets(net.E_constant, net.site_energies[net.N:])
+        b = engine.Batch(engine.make_trajs(1, lay.id, es[0], np.arange(1, dtype=np.uint64), hops,
+                                           net.occupation))
+        try:
+            b.run(precision=_lib.FP64_DENSE)
+            b.sync()
+            b.run(precision=_lib.FP64_DENSE)
+            ms = b.sync()
+            cyc = (C.c_uint64 * 8)()
+            _lib.check(_lib.load().kmc_probe_fp64_phases(b.id, cyc, 8))
+        finally:
+            b.close()
+    finally:
+        lay.close()
+    per = [cyc[k] / hops for k in range(len(FP64_PHASES))]
+    return {"hops": hops, "us_per_hop_events": ms * 1e3 / hops,
+            "cycles_per_hop": dict(zip(FP64_PHASES, [round(x, 1) for x in per])),
+            "cycles_per_hop_total": round(sum(per), 1)}
+
+
+CHAIN_KINDS = ["v_add_f64 dependent (64 lanes)", "v_add_f64 dependent (1 lane)",
+               "v_add_f32 dependent", "v_fma_f64 dependent", "v_add_f64 4 interleaved chains"]
+
+
+def chain_latency(iters: int = 4096) -> dict:
+    """Shader cycles per instruction of dependent FP64/FP32 add chains in one
+    wave (the FP64 prefix chain's bound) and of four interleaved chains."""
+    out = {}
+    for k, name in enumerate(CHAIN_KINDS):
+        c = C.c_double()
+        _lib.check(_lib.load().kmc_probe_chain(k, iters, C.byref(c)))
+        out[name] = round(c.value, 2)
+    return out
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
-    ap.add_argument("what", choices=["copy", "lds", "lone"])
+    ap.add_argument("what", choices=["copy", "lds", "lone", "fp64phases", "chain"])
     ap.add_argument("--bytes", type=int, default=1 << 30)
     ap.add_argument("--repeats", type=int, default=5)
     ap.add_argument("--width", type=int, default=-1, help="4, 8, 16, 0 (stream); -1 all")
@@ -72,6 +124,10 @@ def main() -> None:
         print(json.dumps(copy(args.bytes, args.repeats)))
     elif args.what == "lone":
         print(json.dumps(lone(args.N, args.P, args.form, args.team, args.hops)))
+    elif args.what == "fp64phases":
+        print(json.dumps(fp64_phases(args.hops)))
+    elif args.what == "chain":
+        print(json.dumps(chain_latency()))
     else:
         for w in (4, 8, 16, 0) if args.width < 0 else (args.width,):
             print(json.dumps(lds(w)))
