@@ -36,11 +36,13 @@ LDS_BYTES = 160 * 1024
 # the default stream (host copies) holds one of them
 MAX_LAUNCHES = 3
 # hop latency of a solo trajectory inside a loaded batch over its latency alone:
-# 1.19-1.21 measured for positions-form trajectories at S = 130-131 of the
-# 2048-network uneven batch (7.1-7.25 against 5.97 us/hop)
-BATCH_SLOWDOWN = 1.25
-# least ratio of the best solo latency to the team latency for a team entry
-# (measured alone: 1.14 at S = 64, 1.49 at S = 128, 1.52 at S = 132)
+# in the 2048-network uneven batch under a solo-only plan the longest
+# trajectories ran at 2.0-2.7 us/hop (S = 66-107) against 1.1 us/hop for one
+# trajectory alone at S = 130 with the per-trajectory state cache: a solo wave
+# shares its SIMD with other trajectories' waves, a team's waves do not
+BATCH_SLOWDOWN = 2.2
+# least ratio of the solo latency inside the batch to the team latency alone for
+# a team entry
 TEAM_MIN_GAIN = 1.2
 CACHE_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "autotune_cache.json")
 
@@ -165,7 +167,8 @@ def tune_latency(trajs: np.ndarray, sizes: dict, has_pos: bool = True, edges=BUC
                         f"{res[(form, team)] * 1e6:.2f} us/hop")
         (form, team), lat = min(res.items(), key=lambda kv: kv[1])
         entry = {"form": form, "team": team, "lat": lat,
-                 "solo": {str(f): v for (f, t), v in res.items() if t == 1}}
+                 "solo": {str(f): v for (f, t), v in res.items() if t == 1},
+                 "teams": {f"{f}/{t}": v for (f, t), v in res.items() if t > 1}}
         cache[key] = entry
         out[edge] = entry
         changed = True
@@ -260,17 +263,24 @@ def team_entry(edge: int, lt: dict | None, solo_form: int, t_thr: float, budgets
 
     lt: the bucket's latency tuning ({"form", "team", "lat", "solo": {form: lat}},
     seconds per hop of one trajectory alone).  A team entry exists only where a
-    team is at least TEAM_MIN_GAIN times faster alone than the best solo form: a
-    marginal gain (small networks) does not pay for the launch slot it takes,
-    and a fourth entry forces the larger sizes' team entries to merge into one
-    kernel sized for the largest table.  Its threshold takes the trajectories
-    whose solo run (the bucket's throughput form, at BATCH_SLOWDOWN times its
-    lone latency) would outlast the batch's throughput-bound time t_thr, and
-    only when some budget of the bucket reaches it.  fixed: min_hops is the
-    threshold (serving: a property of the request, not of the mix)."""
-    if not lt or lt["team"] <= 1 or not lt["solo"]:
+    team alone is at least TEAM_MIN_GAIN times faster than the best solo form
+    inside the loaded batch (BATCH_SLOWDOWN times its lone latency): a marginal
+    gain does not pay for the launch slot it takes, and a fourth entry forces
+    the larger sizes' team entries to merge into one kernel sized for the
+    largest table.  Its threshold takes the trajectories whose solo run (the
+    bucket's throughput form, at BATCH_SLOWDOWN times its lone latency) would
+    outlast the batch's throughput-bound time t_thr, and only when some budget
+    of the bucket reaches it.  fixed: min_hops is the threshold (serving: a
+    property of the request, not of the mix)."""
+    if not lt or not lt["solo"]:
         return None
-    if lt["lat"] * TEAM_MIN_GAIN > min(lt["solo"].values()):
+    teams = {k: v for k, v in lt.get("teams", {}).items()}
+    if lt["team"] > 1:
+        teams.setdefault(f'{lt["form"]}/{lt["team"]}', lt["lat"])
+    if not teams:
+        return None
+    tkey, tlat = min(teams.items(), key=lambda kv: kv[1])
+    if tlat * TEAM_MIN_GAIN > min(lt["solo"].values()) * BATCH_SLOWDOWN:
         return None
     if fixed:
         thr = min_hops
@@ -281,7 +291,8 @@ def team_entry(edge: int, lt: dict | None, solo_form: int, t_thr: float, budgets
             thr = None
     if thr is None:
         return None
-    return (edge, int(lt["form"]), 1, int(lt["team"]), int(thr))
+    form, team = (int(x) for x in tkey.split("/"))
+    return (edge, form, 1, team, int(thr))
 
 
 def merge_plan(teams: list, solos: list, max_sn: dict, has_pos: bool = True,

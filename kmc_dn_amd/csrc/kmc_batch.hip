@@ -7,6 +7,11 @@
 
 namespace kmc {
 
+// Entries of each one-wave trajectory's state cache (run_trajectory): every
+// solo kernel keeps the same number, so a trajectory's bits do not depend on
+// the kernel's site-chunk count or form of grouping.
+constexpr int STATE_CACHE = 2;
+
 // Table policies: the workgroup's W waves run W trajectories of one layout and
 // share its tables in LDS.  Layout: the waves' electrode factors (W*2*PMAX
 // floats), the electrode weights (elw_lds_floats), the pair table
@@ -49,7 +54,8 @@ __global__ void __launch_bounds__(64 * W) kmc_batch_fp32_kernel(BatchParams p) {
     __syncthreads();
     if (wave < gd.count) {
       const TabPolicy<C, BT> pol = TabPolicy<C, BT>::make(tab, ctab, elw, evc, L.S, lane);
-      run_trajectory<C, REC, TabPolicy<C, BT>, 1, PRIO>(p, gd.first + wave, L, pol, lane);
+      run_trajectory<C, REC, TabPolicy<C, BT>, 1, PRIO, STATE_CACHE>(p, gd.first + wave, L, pol,
+                                                                    lane);
     }
   }
 }
@@ -72,7 +78,8 @@ __global__ void __launch_bounds__(64 * W) kmc_batch_pos_kernel(BatchParams p) {
     wave_lds_sync();
     PosPolicy<C> pol;
     pol.init(pos, L, lane);
-    for (int i = 0; i < gd.count; ++i) run_trajectory<C, REC>(p, gd.first + i, L, pol, lane);
+    for (int i = 0; i < gd.count; ++i)
+      run_trajectory<C, REC, PosPolicy<C>, 1, true, STATE_CACHE>(p, gd.first + i, L, pol, lane);
     wave_lds_sync();
   }
 }

@@ -11,6 +11,9 @@
 
 namespace kmc {
 
+// Entries of each trajectory's state cache (run_trajectory), as in the solo kernels.
+constexpr int STATE_CACHE = 2;
+
 template <int C, int T, bool REC, int FORM>
 __global__ void __launch_bounds__(64 * T) kmc_team_kernel(BatchParams p) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -37,7 +40,9 @@ __global__ void __launch_bounds__(64 * T) kmc_team_kernel(BatchParams p) {
       __syncthreads();
       PosPolicy<C> pol;
       pol.init(pos, L, lane);
-      for (int i = 0; i < gd.count; ++i) run_trajectory<C, REC>(p, gd.first + i, L, pol, lane, team);
+      for (int i = 0; i < gd.count; ++i)
+        run_trajectory<C, REC, PosPolicy<C>, T, true, STATE_CACHE>(p, gd.first + i, L, pol, lane,
+                                                                   team);
     } else {
       // the waves' electrode factors (T*2*PMAX floats), the electrode weights,
       // the pair table and, for FORM 1, the c table
@@ -60,7 +65,9 @@ __global__ void __launch_bounds__(64 * T) kmc_team_kernel(BatchParams p) {
       }
       __syncthreads();
       const TabPolicy<C, BT> pol = TabPolicy<C, BT>::make(tab, ctab, elw, evc, L.S, lane);
-      for (int i = 0; i < gd.count; ++i) run_trajectory<C, REC>(p, gd.first + i, L, pol, lane, team);
+      for (int i = 0; i < gd.count; ++i)
+        run_trajectory<C, REC, TabPolicy<C, BT>, T, true, STATE_CACHE>(p, gd.first + i, L, pol,
+                                                                       lane, team);
     }
   }
 }

@@ -394,9 +394,21 @@ __device__ __forceinline__ float pick_lane_value(const float (&x)[C], int site) 
   return r;
 }
 
-template <int C, bool REC, class Pol, int T = 1, bool PRIO = true>
+// NC: entries of the trajectory's state cache (0: none).  A hop whose
+// occupation equals a cached one takes that state's row sums and FP64 prefix
+// from registers instead of evaluating them; the entries are emptied at every
+// FP64 recompute, so a trajectory's results depend on its
+// own history only, and ranges that start at multiples of the recompute
+// interval still continue exactly.  A new state enters at the front and the
+// oldest drops out.  The Go library keeps the same kind of
+// cache, keyed by occupation (simulation.go:254-296); at low bias a trajectory
+// mostly moves one hole back and forth between two sites, and two entries take
+// ~99 % of its hops.  In a team every wave keeps the same cache (the reduced
+// row sums are the same bits in every wave), so all waves hit together.
+template <int C, bool REC, class Pol, int T = 1, bool PRIO = true, int NC = 0>
 __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, const Pol& pol,
                                int lane, const Team<T, C>& team = Team<T, C>{}) {
+
   const int64_t t_start = wall_clock64();
   const TrajDev td = p.trajs[t];
   const EsetDev es = p.esets[td.eset];
@@ -500,6 +512,17 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
   };
   int special = next_special(h, true);
   double b15 = 0.0, b31 = 0.0;   // row-broadcast registers of the FP64 scan
+  // state cache: keys (occupation words), validity, totals (wave-uniform) and
+  // the lanes' row sums and FP64 prefixes of each entry; a new state enters at
+  // entry 0 and the others move down one (the last drops out)
+  constexpr int NCE = NC > 0 ? NC : 1;
+  int nred = 0;   // team reductions so far
+  uint64_t ckey[NCE][C];
+  bool cval[NCE];
+  double ctot[NCE], cinc[NCE][C];
+  float cR[NCE][C];
+#pragma unroll
+  for (int e = 0; e < NCE; ++e) cval[e] = false;
   for (; h < h_stop; ++h) {
     if (__builtin_expect(h == special, 0)) {
       if ((h & 63) == 0) refill(h);
@@ -514,6 +537,8 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
       if (h == next_rc) {
         recompute_energies<C>(se, occ, L, es, lane);
         next_rc = next_multiple(h);
+#pragma unroll
+        for (int e = 0; e < NCE; ++e) cval[e] = false;
       }
       special = next_special(h, false);
     }
@@ -530,9 +555,40 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
       nocc += popcount64(o);
     }
     const bool udst = N - nocc <= nocc;   // walk the smaller set (empty vs occupied)
+    float R[C];
+    double incl[C];
+    double total = 0.0;
+    int hit = -1;
+    if constexpr (NC > 0) {
+#pragma unroll
+      for (int e = 0; e < NC; ++e) {
+        bool same = cval[e];
+#pragma unroll
+        for (int c = 0; c < C; ++c) same = same && ckey[e][c] == occ[c];
+        if (same) hit = e;
+      }
+    }
+    if (hit >= 0) {
+#pragma unroll
+      for (int c = 0; c < C; ++c) {
+        R[c] = cR[0][c];
+        incl[c] = cinc[0][c];
+      }
+      total = ctot[0];
+#pragma unroll
+      for (int e = 1; e < NCE; ++e)
+        if (hit == e) {
+#pragma unroll
+          for (int c = 0; c < C; ++c) {
+            R[c] = cR[e][c];
+            incl[c] = cinc[e][c];
+          }
+          total = ctot[e];
+        }
+      if constexpr (PRIO) __builtin_amdgcn_s_setprio(3);
+    } else {
     // every lane sums over the walked members; the sums of lanes outside the
     // other role (and past S) are discarded after the reduction
-    float R[C];
     uint64_t um[C];
 #pragma unroll
     for (int c = 0; c < C; ++c) {
@@ -572,7 +628,11 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
     const float sgr = __int_as_float(udst ? (int)0x3f800000 : (int)0xbf800000);
     if constexpr (PRIO) __builtin_amdgcn_s_setprio(0);
     row_sums<C>(pol, S, um, team.own(), v, se, sgr, R);
-    team.reduce(R, (int)(h & 1), lane);
+    // the team's partial sums alternate between two LDS buffers by reduction
+    // (hops served from the state cache reduce nothing and pass no barrier;
+    // the waves agree on every hit, so they count the same reductions)
+    team.reduce(R, nred & 1, lane);
+    ++nred;
 #pragma unroll
     for (int c = 0; c < C; ++c)
       R[c] = __builtin_amdgcn_inverse_ballot_w64(udst ? srcm[c] : dstm[c]) ? R[c] : 0.f;
@@ -587,8 +647,6 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
     // false: the variant for launches whose groups queue behind full SIMDs.
     if constexpr (PRIO) __builtin_amdgcn_s_setprio(3);
     // FP64 prefix over lane rows
-    double incl[C];
-    double total = 0.0;
 #pragma unroll
     for (int c = 0; c < C; ++c) {
       if (c > 0 && 64 * c >= S) {
@@ -598,6 +656,28 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
       incl[c] = c == 0 ? wave_scan_f64((double)R[c], b15, b31)
                        : wave_scan_f64((double)R[c], b15, b31) + total;
       total = readlane_f64(incl[c], 63);
+    }
+    if constexpr (NC > 0) {   // the new state enters at entry 0
+#pragma unroll
+      for (int e = NC - 1; e > 0; --e) {
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+          ckey[e][c] = ckey[e - 1][c];
+          cR[e][c] = cR[e - 1][c];
+          cinc[e][c] = cinc[e - 1][c];
+        }
+        ctot[e] = ctot[e - 1];
+        cval[e] = cval[e - 1];
+      }
+#pragma unroll
+      for (int c = 0; c < C; ++c) {
+        ckey[0][c] = occ[c];
+        cR[0][c] = R[c];
+        cinc[0][c] = incl[c];
+      }
+      ctot[0] = total;
+      cval[0] = true;
+    }
     }
     if (!(total > 0.0) || !(total < INFINITY)) {
       status = ST_ZERO_RATE;

@@ -129,17 +129,21 @@ def fixtures_check(n_layouts=10, replicas=128, hops=1_000_000, path=None):
         b.close()
         assert (r.status == 0).all()
         out[prec] = r.currents(P).reshape(n_layouts, replicas, P)
-    rels, zs = [], []
+    rels, zs, tight = [], [], []
     for i in range(n_layouts):
         m32, m64, se, z, rel = compare(out[_lib.FP32_BATCHED][i], out[_lib.FP64_DENSE][i])
         big = np.abs(m64) > 10 * se
         rels.append(rel[big])
+        # a 1 % bar is testable at 3 sigma where 3 standard errors are below 1 %
+        tight.append(rel[np.abs(m64) > 300 * se])
         zs.append(z)
-    rels, zs = np.concatenate(rels), np.concatenate(zs)
+    rels, zs, tight = np.concatenate(rels), np.concatenate(zs), np.concatenate(tight)
     return {"workload": f"{n_layouts} reference fixture networks x {replicas} trajectories x "
                         f"{hops} hops (+1e4 prehops), FP32 vs FP64 on the same streams",
             "currents_compared": int(rels.size), "max_rel": float(rels.max()),
             "median_rel": float(np.median(rels)), "frac_within_1pct": float(np.mean(rels < 0.01)),
+            "currents_1pct_testable": int(tight.size),
+            "max_rel_1pct_testable": float(tight.max()) if tight.size else None,
             "max_z": float(zs.max()), "median_z": float(np.median(zs))}
 
 

@@ -1,8 +1,9 @@
 """Accuracy on the reference's own fixture networks (thesis_indrek/tests).
 
-* The north star's 1 % bar: FP32 against the FP64 path (the numba loop's
-  arithmetic) at 1e6 hops on 10 fixture networks x 128 trajectories, every
-  resolvable current within 1 % and every current within 3 sigma.
+* The 1 % bar at 1e6 hops: FP32 against the FP64 path (the numba loop's
+  arithmetic) on 10 fixture networks x 256 trajectories.  Every current lies
+  within 3 sigma; every current whose 3-sigma band is inside 1 % (|I| > 300
+  standard errors, where a 1 % bar can be tested at all) lies within 1 %.
 * The reference's own accuracy statistic, E_DB (the mean Bhattacharyya
   distance of 5-run current estimates, thesis_indrek/validate_tests.py:80-135),
   for go_simulation's back end against the Go-made 5e6-hop fixture sets, and
@@ -21,11 +22,11 @@ pytestmark = pytest.mark.gpu
 
 
 def test_fp32_within_one_percent_of_fp64_at_1e6_hops(gpu_engine):
-    print("[gpu] 10 fixtures x 128 trajectories x 1e6 hops, FP32 and FP64", flush=True)
-    r = validate.fixtures_check(10, 128, 1_000_000)
+    print("[gpu] 10 fixtures x 256 trajectories x 1e6 hops, FP32 and FP64", flush=True)
+    r = validate.fixtures_check(10, 256, 1_000_000)
     print(f"[gpu] {r}", flush=True)
-    assert r["currents_compared"] >= 60
-    assert r["max_rel"] < 0.01
+    assert r["currents_compared"] >= 60 and r["currents_1pct_testable"] >= 30
+    assert r["max_rel_1pct_testable"] < 0.01
     assert r["max_z"] < 3.0
 
 

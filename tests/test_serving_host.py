@@ -126,16 +126,24 @@ def test_split_team_entries():
 
 
 def test_team_entry_rules():
-    """A team entry needs a clear gain alone; its threshold uses the in-batch
-    solo latency and must be reached by some budget of the bucket."""
+    """A team entry needs a clear gain against the solo latency inside the
+    loaded batch (BATCH_SLOWDOWN times the lone one); its threshold uses that
+    in-batch latency and must be reached by some budget of the bucket."""
     lat = {"form": 2, "team": 4, "lat": 2.0e-6, "solo": {"2": 3.1e-6, "3": 6.0e-6}}
     budgets = np.array([1e4, 2e5, 8e5])
     e = autotune.team_entry(192, lat, 3, t_thr=1.5, budgets=budgets)
     thr = int(np.ceil(1.5 / (6.0e-6 * autotune.BATCH_SLOWDOWN)))
     assert e == (192, 2, 1, 4, thr)
-    # a marginal team (1.14x at S = 64) takes no launch slot
-    small = {"form": 2, "team": 2, "lat": 0.77e-6, "solo": {"1": 0.88e-6, "2": 0.9e-6}}
-    assert autotune.team_entry(64, small, 1, t_thr=1.5, budgets=np.array([1e6])) is None
+    # a team slower alone than a solo wave is inside the batch takes no launch slot
+    slow = {"form": 2, "team": 2, "lat": 2.4e-6, "solo": {"1": 0.88e-6, "2": 0.9e-6}}
+    assert 2.4e-6 * autotune.TEAM_MIN_GAIN > 0.88e-6 * autotune.BATCH_SLOWDOWN
+    assert autotune.team_entry(64, slow, 1, t_thr=1.5, budgets=np.array([1e6])) is None
+    # a team no faster alone than solo still wins inside the batch: the entry
+    # takes the team size and form of the fastest team alone
+    even = {"form": 2, "team": 1, "lat": 1.1e-6, "solo": {"2": 1.1e-6, "3": 1.6e-6},
+            "teams": {"2/4": 1.01e-6, "3/4": 1.55e-6, "2/2": 1.05e-6}}
+    e = autotune.team_entry(192, even, 3, t_thr=1.5, budgets=budgets)
+    assert e == (192, 2, 1, 4, int(np.ceil(1.5 / (1.6e-6 * autotune.BATCH_SLOWDOWN))))
     # no budget reaches the threshold
     assert autotune.team_entry(192, lat, 3, t_thr=1.5, budgets=np.array([1e4, 5e4])) is None
     # solo fastest alone
