@@ -36,14 +36,23 @@ LDS_BYTES = 160 * 1024
 # the default stream (host copies) holds one of them
 MAX_LAUNCHES = 3
 # hop latency of a solo trajectory inside a loaded batch over its latency alone:
-# in the 2048-network uneven batch under a solo-only plan the longest
-# trajectories ran at 2.0-2.7 us/hop (S = 66-107) against 1.1 us/hop for one
-# trajectory alone at S = 130 with the per-trajectory state cache: a solo wave
-# shares its SIMD with other trajectories' waves, a team's waves do not
-BATCH_SLOWDOWN = 2.2
+# in the 2048-network uneven batch with the per-trajectory state cache, an
+# S = 130 trajectory in the positions-form solo entry ran at 4.19 us/hop against
+# 1.33 alone (3.15x), and under a table-form solo-only plan the longest
+# trajectories ran at 2.0-2.7 us/hop (S = 66-107) against 1.1 alone: a solo
+# wave shares its SIMD with other trajectories' waves, a team's waves do not
+BATCH_SLOWDOWN = 3.0
 # least ratio of the solo latency inside the batch to the team latency alone for
 # a team entry
 TEAM_MIN_GAIN = 1.2
+# team size of team entries: four waves fill a CU's four SIMDs, and a table
+# large enough to need two or more site chunks keeps other workgroups off most
+# of the CU's LDS.  With the state cache, teams of 1, 2 and 4 waves are within
+# 5 % of one another alone (0.96-0.99 us/hop at S = 132); in the batch, teams of
+# 4 for S > 64 gave 1.15 s and an autotuned plan of teams of 2 with a
+# one-chunk team entry 1.89 s.
+TEAM_SIZE = 4
+TEAM_MIN_EDGE = 128
 CACHE_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "autotune_cache.json")
 
 
@@ -261,9 +270,11 @@ def team_entry(edge: int, lt: dict | None, solo_form: int, t_thr: float, budgets
                min_hops: int | None = None, fixed: bool = False):
     """The team entry of one size bucket, or None.
 
-    lt: the bucket's latency tuning ({"form", "team", "lat", "solo": {form: lat}},
-    seconds per hop of one trajectory alone).  A team entry exists only where a
-    team alone is at least TEAM_MIN_GAIN times faster than the best solo form
+    lt: the bucket's latency tuning ({"form", "team", "lat", "solo": {form: lat},
+    "teams": {"form/team": lat}}, seconds per hop of one trajectory alone).  Only
+    buckets of two or more site chunks get a team entry, of TEAM_SIZE waves in
+    the form fastest alone at that size.  It exists only where that team alone
+    is at least TEAM_MIN_GAIN times faster than the best solo form
     inside the loaded batch (BATCH_SLOWDOWN times its lone latency): a marginal
     gain does not pay for the launch slot it takes, and a fourth entry forces
     the larger sizes' team entries to merge into one kernel sized for the
@@ -272,11 +283,13 @@ def team_entry(edge: int, lt: dict | None, solo_form: int, t_thr: float, budgets
     outlast the batch's throughput-bound time t_thr, and only when some budget
     of the bucket reaches it.  fixed: min_hops is the threshold (serving: a
     property of the request, not of the mix)."""
-    if not lt or not lt["solo"]:
+    if not lt or not lt["solo"] or edge < TEAM_MIN_EDGE:
         return None
     teams = {k: v for k, v in lt.get("teams", {}).items()}
     if lt["team"] > 1:
         teams.setdefault(f'{lt["form"]}/{lt["team"]}', lt["lat"])
+    sized = {k: v for k, v in teams.items() if int(k.split("/")[1]) == TEAM_SIZE}
+    teams = sized or teams
     if not teams:
         return None
     tkey, tlat = min(teams.items(), key=lambda kv: kv[1])

@@ -135,15 +135,17 @@ def test_team_entry_rules():
     thr = int(np.ceil(1.5 / (6.0e-6 * autotune.BATCH_SLOWDOWN)))
     assert e == (192, 2, 1, 4, thr)
     # a team slower alone than a solo wave is inside the batch takes no launch slot
-    slow = {"form": 2, "team": 2, "lat": 2.4e-6, "solo": {"1": 0.88e-6, "2": 0.9e-6}}
-    assert 2.4e-6 * autotune.TEAM_MIN_GAIN > 0.88e-6 * autotune.BATCH_SLOWDOWN
-    assert autotune.team_entry(64, slow, 1, t_thr=1.5, budgets=np.array([1e6])) is None
+    slow = {"form": 2, "team": 4, "lat": 4.4e-6, "solo": {"1": 1.38e-6, "2": 1.4e-6}}
+    assert 4.4e-6 * autotune.TEAM_MIN_GAIN > 1.38e-6 * autotune.BATCH_SLOWDOWN
+    assert autotune.team_entry(128, slow, 1, t_thr=1.5, budgets=np.array([1e6])) is None
+    # one-chunk buckets take no team entry
+    assert autotune.team_entry(64, lat, 3, t_thr=1.5, budgets=budgets) is None
     # a team no faster alone than solo still wins inside the batch: the entry
-    # takes the team size and form of the fastest team alone
-    even = {"form": 2, "team": 1, "lat": 1.1e-6, "solo": {"2": 1.1e-6, "3": 1.6e-6},
-            "teams": {"2/4": 1.01e-6, "3/4": 1.55e-6, "2/2": 1.05e-6}}
+    # takes teams of TEAM_SIZE waves in the form fastest alone at that size
+    even = {"form": 2, "team": 2, "lat": 0.96e-6, "solo": {"2": 0.98e-6, "3": 1.33e-6},
+            "teams": {"2/4": 0.99e-6, "3/4": 1.37e-6, "2/2": 0.96e-6}}
     e = autotune.team_entry(192, even, 3, t_thr=1.5, budgets=budgets)
-    assert e == (192, 2, 1, 4, int(np.ceil(1.5 / (1.6e-6 * autotune.BATCH_SLOWDOWN))))
+    assert e == (192, 2, 1, 4, int(np.ceil(1.5 / (1.33e-6 * autotune.BATCH_SLOWDOWN))))
     # no budget reaches the threshold
     assert autotune.team_entry(192, lat, 3, t_thr=1.5, budgets=np.array([1e4, 5e4])) is None
     # solo fastest alone
