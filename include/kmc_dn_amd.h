@@ -348,6 +348,22 @@ int kmc_probe_chain(int kind, int iters, double* cycles_per_instruction);
  * hop and records (n >= 7). */
 int kmc_probe_fp64_phases(int32_t batch, uint64_t* cycles, int32_t n);
 
+/* Runs the batch once as kmc_batch_run(batch, opts) would on the FP32 batched
+ * path, with every launch on its kernel's phase-timing instantiation
+ * (shader-clock reads at the phase boundaries of every hop), and returns the
+ * shader cycles of each trajectory's first wave summed over trajectories:
+ * cycles[0..6] = uniforms and FP64 recomputes, a miss's state-cache lookup,
+ * energy update, electrode terms, row sums and team reduction, a miss's FP64
+ * prefix scan and cache insert, clock and event selection, partner selection,
+ * hop (occupation and tallies), a hit's lookup and copy; cycles[7] = the
+ * number of state-cache hits, cycles[8] = the number of hops whose partner
+ * prefix came from the state's entry (n >= 9).
+ * Solo launches need a table form and 4 waves per workgroup, team launches
+ * teams of 4 on 2 or 3 site chunks; no records, graph or resume.  The clock
+ * reads wait for outstanding loads, so the instrumented hops run slower than
+ * the ordinary kernel's. */
+int kmc_probe_fp32_phases(int32_t batch, const kmc_run_opts* opts, uint64_t* cycles, int32_t n);
+
 #ifdef __cplusplus
 }
 #endif

@@ -97,6 +97,7 @@ SIGNATURES = {
     "kmc_probe_copy": (C.c_int, [_i64, C.c_int, _dp]),
     "kmc_probe_lds": (C.c_int, [C.c_int, C.c_int, C.c_int, _dp]),
     "kmc_probe_fp64_phases": (C.c_int, [_i32, _u64p, _i32]),
+    "kmc_probe_fp32_phases": (C.c_int, [_i32, C.POINTER(kmc_run_opts), _u64p, _i32]),
     "kmc_probe_chain": (C.c_int, [C.c_int, C.c_int, _dp]),
 }
 

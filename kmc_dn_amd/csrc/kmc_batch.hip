@@ -7,16 +7,12 @@
 
 namespace kmc {
 
-// Entries of each one-wave trajectory's state cache (run_trajectory): every
-// solo kernel keeps the same number, so a trajectory's bits do not depend on
-// the kernel's site-chunk count or form of grouping.
-constexpr int STATE_CACHE = 2;
-
 // Table policies: the workgroup's W waves run W trajectories of one layout and
 // share its tables in LDS.  Layout: the waves' electrode factors (W*2*PMAX
 // floats), the electrode weights (elw_lds_floats), the pair table
-// (S*S + 64C float2) and, for BT, the c table (N*S + 64C floats).
-template <int C, int W, bool REC, bool BT, bool PRIO>
+// (S*S + 64C float2) and, for BT, the c table (N*S + 64C floats).  PH: the
+// phase-timing instantiation (run_trajectory).
+template <int C, int W, bool REC, bool BT, bool PRIO, bool PH>
 __global__ void __launch_bounds__(64 * W) kmc_batch_fp32_kernel(BatchParams p) {
   extern __shared__ __align__(16) unsigned char smem[];
   // wave index and everything read from the shared group slot are wave-uniform:
@@ -54,8 +50,8 @@ __global__ void __launch_bounds__(64 * W) kmc_batch_fp32_kernel(BatchParams p) {
     __syncthreads();
     if (wave < gd.count) {
       const TabPolicy<C, BT> pol = TabPolicy<C, BT>::make(tab, ctab, elw, evc, L.S, lane);
-      run_trajectory<C, REC, TabPolicy<C, BT>, 1, PRIO, STATE_CACHE>(p, gd.first + wave, L, pol,
-                                                                    lane);
+      run_trajectory<C, REC, TabPolicy<C, BT>, 1, PRIO, PH>(p, gd.first + wave, L,
+                                                                        pol, lane);
     }
   }
 }
@@ -79,23 +75,29 @@ __global__ void __launch_bounds__(64 * W) kmc_batch_pos_kernel(BatchParams p) {
     PosPolicy<C> pol;
     pol.init(pos, L, lane);
     for (int i = 0; i < gd.count; ++i)
-      run_trajectory<C, REC, PosPolicy<C>, 1, true, STATE_CACHE>(p, gd.first + i, L, pol, lane);
+      run_trajectory<C, REC, PosPolicy<C>, 1, true>(p, gd.first + i, L, pol, lane);
     wave_lds_sync();
   }
 }
 
-#define KMC_INST(C, W)                                                                    \
-  template __global__ void kmc_batch_fp32_kernel<C, W, false, false, true>(BatchParams);  \
-  template __global__ void kmc_batch_fp32_kernel<C, W, true, false, true>(BatchParams);   \
-  template __global__ void kmc_batch_fp32_kernel<C, W, false, true, true>(BatchParams);   \
-  template __global__ void kmc_batch_fp32_kernel<C, W, true, true, true>(BatchParams);    \
-  template __global__ void kmc_batch_fp32_kernel<C, W, false, false, false>(BatchParams); \
-  template __global__ void kmc_batch_fp32_kernel<C, W, false, true, false>(BatchParams);  \
-  template __global__ void kmc_batch_pos_kernel<C, W, false>(BatchParams);         \
+#define KMC_INST(C, W)                                                                           \
+  template __global__ void kmc_batch_fp32_kernel<C, W, false, false, true, false>(BatchParams);  \
+  template __global__ void kmc_batch_fp32_kernel<C, W, true, false, true, false>(BatchParams);   \
+  template __global__ void kmc_batch_fp32_kernel<C, W, false, true, true, false>(BatchParams);   \
+  template __global__ void kmc_batch_fp32_kernel<C, W, true, true, true, false>(BatchParams);    \
+  template __global__ void kmc_batch_fp32_kernel<C, W, false, false, false, false>(BatchParams); \
+  template __global__ void kmc_batch_fp32_kernel<C, W, false, true, false, false>(BatchParams);  \
+  template __global__ void kmc_batch_pos_kernel<C, W, false>(BatchParams);                       \
   template __global__ void kmc_batch_pos_kernel<C, W, true>(BatchParams);
 KMC_INST(1, 1) KMC_INST(1, 2) KMC_INST(1, 4) KMC_INST(1, 8)
 KMC_INST(2, 1) KMC_INST(2, 2) KMC_INST(2, 4) KMC_INST(2, 8)
 KMC_INST(3, 1) KMC_INST(3, 2) KMC_INST(3, 4) KMC_INST(3, 8)
+#undef KMC_INST
+// phase timing: 4 waves per workgroup, both table forms, graded issue priority
+#define KMC_INST(C)                                                                           \
+  template __global__ void kmc_batch_fp32_kernel<C, 4, false, false, true, true>(BatchParams); \
+  template __global__ void kmc_batch_fp32_kernel<C, 4, false, true, true, true>(BatchParams);
+KMC_INST(1) KMC_INST(2) KMC_INST(3)
 #undef KMC_INST
 
 // Zeroes the work-queue head (a kernel node, so a captured graph can replay it).

@@ -22,7 +22,7 @@ namespace kmc {
 
 template <bool REC, bool STAGE, bool PHASE>
 __global__ void kmc_plain_fp64_kernel(PlainParams);
-template <int C, int W, bool REC, bool BT, bool PRIO>
+template <int C, int W, bool REC, bool BT, bool PRIO, bool PH>
 __global__ void kmc_batch_fp32_kernel(BatchParams);
 template <int C, int W, bool REC>
 __global__ void kmc_batch_pos_kernel(BatchParams);
@@ -33,7 +33,7 @@ __global__ void kmc_eset_scatter_kernel(const EsetDev*, const int32_t*, const in
 size_t plain_lds_bytes(int S, int N, bool stage);
 size_t batch_lds_bytes(bool bt, int C, int W, int S, int N, int P);
 size_t pos_lds_bytes(int C, int W);
-template <int C, int T, bool REC, int FORM>
+template <int C, int T, bool REC, int FORM, bool PH>
 __global__ void kmc_team_kernel(BatchParams);
 size_t team_lds_bytes(int form, int C, int T, int S, int N, int P);
 template <int C>
@@ -799,12 +799,12 @@ BatchKernel pick_batch_kernel(int C, int W, bool rec, bool bt, bool prio) {
 #define KMC_PICK(CC, WW)                                                                        \
   if (C == CC && W == WW) {                                                                     \
     if (!rec && !prio)                                                                          \
-      return bt ? (BatchKernel)kmc_batch_fp32_kernel<CC, WW, false, true, false>                \
-                : (BatchKernel)kmc_batch_fp32_kernel<CC, WW, false, false, false>;              \
-    if (bt) return rec ? (BatchKernel)kmc_batch_fp32_kernel<CC, WW, true, true, true>           \
-                       : (BatchKernel)kmc_batch_fp32_kernel<CC, WW, false, true, true>;         \
-    return rec ? (BatchKernel)kmc_batch_fp32_kernel<CC, WW, true, false, true>                  \
-               : (BatchKernel)kmc_batch_fp32_kernel<CC, WW, false, false, true>;                \
+      return bt ? (BatchKernel)kmc_batch_fp32_kernel<CC, WW, false, true, false, false>         \
+                : (BatchKernel)kmc_batch_fp32_kernel<CC, WW, false, false, false, false>;       \
+    if (bt) return rec ? (BatchKernel)kmc_batch_fp32_kernel<CC, WW, true, true, true, false>    \
+                       : (BatchKernel)kmc_batch_fp32_kernel<CC, WW, false, true, true, false>;  \
+    return rec ? (BatchKernel)kmc_batch_fp32_kernel<CC, WW, true, false, true, false>           \
+               : (BatchKernel)kmc_batch_fp32_kernel<CC, WW, false, false, true, false>;         \
   }
   KMC_PICK(1, 1) KMC_PICK(1, 2) KMC_PICK(1, 4) KMC_PICK(1, 8)
   KMC_PICK(2, 1) KMC_PICK(2, 2) KMC_PICK(2, 4) KMC_PICK(2, 8)
@@ -816,13 +816,32 @@ BatchKernel pick_batch_kernel(int C, int W, bool rec, bool bt, bool prio) {
 BatchKernel pick_team_kernel(int C, int T, bool rec, int form) {
 #define KMC_PICK(CC, TT, FF)                                                         \
   if (C == CC && T == TT && form == FF)                                              \
-    return rec ? (BatchKernel)kmc_team_kernel<CC, TT, true, FF>                      \
-               : (BatchKernel)kmc_team_kernel<CC, TT, false, FF>;
+    return rec ? (BatchKernel)kmc_team_kernel<CC, TT, true, FF, false>               \
+               : (BatchKernel)kmc_team_kernel<CC, TT, false, FF, false>;
   KMC_PICK(1, 2, 1) KMC_PICK(1, 2, 2) KMC_PICK(1, 2, 3) KMC_PICK(1, 4, 1) KMC_PICK(1, 4, 2)
   KMC_PICK(1, 4, 3) KMC_PICK(2, 2, 1) KMC_PICK(2, 2, 2) KMC_PICK(2, 2, 3) KMC_PICK(2, 4, 1)
   KMC_PICK(2, 4, 2) KMC_PICK(2, 4, 3) KMC_PICK(3, 2, 1) KMC_PICK(3, 2, 2) KMC_PICK(3, 2, 3)
   KMC_PICK(3, 4, 1) KMC_PICK(3, 4, 2) KMC_PICK(3, 4, 3)
 #undef KMC_PICK
+  return nullptr;
+}
+
+// Phase-timing instantiations: solo table kernels of 4 waves, teams of 4 on
+// two- and three-chunk networks (no records, graded issue priority).
+BatchKernel pick_phase_kernel(int C, int W, int team, int form) {
+  if (team == 4 && (C == 2 || C == 3) && form >= 1 && form <= 3) {
+#define KMC_PICK(CC, FF) \
+  if (C == CC && form == FF) return (BatchKernel)kmc_team_kernel<CC, 4, false, FF, true>;
+    KMC_PICK(2, 1) KMC_PICK(2, 2) KMC_PICK(2, 3) KMC_PICK(3, 1) KMC_PICK(3, 2) KMC_PICK(3, 3)
+#undef KMC_PICK
+  }
+  if (team <= 1 && W == 4 && (form == 1 || form == 2)) {
+#define KMC_PICK(CC)                                                                  \
+  if (C == CC) return form == 1 ? (BatchKernel)kmc_batch_fp32_kernel<CC, 4, false, true, true, true> \
+                                : (BatchKernel)kmc_batch_fp32_kernel<CC, 4, false, false, true, true>;
+    KMC_PICK(1) KMC_PICK(2) KMC_PICK(3)
+#undef KMC_PICK
+  }
   return nullptr;
 }
 
@@ -847,9 +866,10 @@ struct Fp32Launch {
 
 // One work-queue kernel over `ngroups` groups.  form: 1 {lt, lt+c} + c table,
 // 2 {lt, c} table, 3 positions; smax/nmax bound the S and N of its layouts.
+// phases: run the phase-timing instantiation, summing into phases[FP32_PHASES].
 int setup_launch(Batch* B, const kmc_run_opts& o, int form, int W, int team, int smax, int nmax,
                  int pmax, const DevBuf& groups, int32_t ngroups, int32_t* queue, int grid_cap,
-                 Fp32Launch& L) {
+                 Fp32Launch& L, uint64_t* phases) {
   Engine& E = eng();
   const int C = (smax + 63) / 64;
   // recording kernels: occupation time (bit 0) and traffic (bit 1); the others
@@ -870,6 +890,15 @@ int setup_launch(Batch* B, const kmc_run_opts& o, int form, int W, int team, int
     lds = batch_lds_bytes(form == 1, C, W, smax, nmax, pmax);
   }
   if (!k) return fail(KMC_ERR_LIMIT, "no kernel for %d site chunks", C);
+  if (phases) {
+    if (rec) return fail(KMC_ERR_UNSUPPORTED, "phase timing runs without records");
+    k = pick_phase_kernel(C, W, team, form);
+    if (!k)
+      return fail(KMC_ERR_UNSUPPORTED,
+                  "no phase-timing kernel for %d site chunks, %d waves, team %d, table form %d "
+                  "(solo: table forms with 4 waves; teams: 4 waves on 2 or 3 chunks)",
+                  C, W, team, form);
+  }
   if (lds > lds_cap)
     return fail(KMC_ERR_LIMIT, "LDS %zu bytes for S=%d exceeds 160 KiB (table form %d)", lds, smax, form);
   HIPOK(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
@@ -879,7 +908,7 @@ int setup_launch(Batch* B, const kmc_run_opts& o, int form, int W, int team, int
   // The graded issue priority of the hop pays while every group is resident at
   // once (4096 trajectories: +12 %); with groups queued behind full SIMDs the
   // variant without it is 2 % faster (65,536-trajectory IV sweep).
-  if (team <= 1 && form != 3 && !rec && ngroups > per_cu * E.cu_count) {
+  if (team <= 1 && form != 3 && !rec && !phases && ngroups > per_cu * E.cu_count) {
     k = pick_batch_kernel(C, W, rec, form == 1, false);
     HIPOK(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     HIPOK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k, 64 * W, lds));
@@ -903,6 +932,7 @@ int setup_launch(Batch* B, const kmc_run_opts& o, int form, int W, int team, int
   p.traffic_stride = (int64_t)B->smax * B->smax;
   p.post_hop_record = o.post_hop_record;
   p.resume = o.resume ? 1 : 0;
+  p.phases = phases;
   L.k = k;
   L.grid = grid;
   L.block = 64 * W;
@@ -978,7 +1008,8 @@ int build_bucket_groups(Batch* B) {
 }
 
 // Everything that allocates or queries happens here, outside any graph capture.
-int prepare_fp32(Batch* B, const kmc_run_opts& o, std::vector<Fp32Launch>& out) {
+int prepare_fp32(Batch* B, const kmc_run_opts& o, std::vector<Fp32Launch>& out,
+                 uint64_t* phases = nullptr) {
   int st = prepare_record(B, o);
   if (st) return st;
   int32_t* q = B->queue.as<int32_t>();
@@ -992,7 +1023,8 @@ int prepare_fp32(Batch* B, const kmc_run_opts& o, std::vector<Fp32Launch>& out) 
       if (pb.table_form == 3 && (st = check_positions(B, b))) return st;
       Fp32Launch L;
       st = setup_launch(B, o, pb.table_form, pb.waves_per_group, pb.team, sb.smax, sb.nmax,
-                        sb.pmax, sb.groups, sb.ngroups, q + 16 * b, pb.grid > 0 ? pb.grid : o.grid_limit, L);
+                        sb.pmax, sb.groups, sb.ngroups, q + 16 * b, pb.grid > 0 ? pb.grid : o.grid_limit, L,
+                        phases);
       if (st) return st;
       out.push_back(L);
     }
@@ -1013,7 +1045,7 @@ int prepare_fp32(Batch* B, const kmc_run_opts& o, std::vector<Fp32Launch>& out) 
   if ((st = build_groups(B, (form == 3 || team > 1) ? 1 : W))) return st;
   Fp32Launch L;
   if ((st = setup_launch(B, o, form, W, team, B->smax, B->nmax, B->pmax, B->groups, B->ngroups, q,
-                         o.grid_limit, L)))
+                         o.grid_limit, L, phases)))
     return st;
   out.push_back(L);
   return KMC_OK;
@@ -1464,6 +1496,32 @@ int kmc_probe_fp64_phases(int32_t batch, uint64_t* cycles, int32_t n) {
     return st;
   HIPOK(hipStreamSynchronize(E.stream));
   HIPOK(hipMemcpy(cycles, ph.p, sizeof(uint64_t) * PLAIN_PHASES, hipMemcpyDeviceToHost));
+  B->last_rec = B->last_traffic = false;
+  return KMC_OK;
+}
+
+int kmc_probe_fp32_phases(int32_t batch, const kmc_run_opts* opts, uint64_t* cycles, int32_t n) {
+  Engine& E = eng();
+  std::lock_guard<std::mutex> lk(E.mu);
+  int st = ensure_init();
+  if (st) return st;
+  Batch* B = get_batch(batch);
+  if (!B) return fail(KMC_ERR_ARGS, "bad batch id %d", batch);
+  if (!cycles || n < FP32_PHASES) return fail(KMC_ERR_ARGS, "need %d phase slots", FP32_PHASES);
+  kmc_run_opts o{};
+  if (opts) o = *opts;
+  if (o.precision != KMC_FP32_BATCHED || o.record || o.use_graph || o.resume)
+    return fail(KMC_ERR_UNSUPPORTED, "phase timing: FP32 batched runs without records, graph or resume");
+  if ((st = sync_registry())) return st;
+  DevBuf ph;
+  if ((st = ph.alloc(sizeof(uint64_t) * FP32_PHASES))) return st;
+  HIPOK(hipMemset(ph.p, 0, sizeof(uint64_t) * FP32_PHASES));
+  std::vector<Fp32Launch> Ls;
+  if ((st = prepare_fp32(B, o, Ls, ph.as<uint64_t>()))) return st;
+  if (Ls.size() > 1 && (st = ensure_side_streams(B, (int)Ls.size()))) return st;
+  if ((st = launch_fp32(B, o, Ls, E.stream))) return st;
+  HIPOK(hipStreamSynchronize(E.stream));
+  HIPOK(hipMemcpy(cycles, ph.p, sizeof(uint64_t) * FP32_PHASES, hipMemcpyDeviceToHost));
   B->last_rec = B->last_traffic = false;
   return KMC_OK;
 }

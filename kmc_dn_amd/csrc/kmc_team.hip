@@ -11,10 +11,8 @@
 
 namespace kmc {
 
-// Entries of each trajectory's state cache (run_trajectory), as in the solo kernels.
-constexpr int STATE_CACHE = 2;
-
-template <int C, int T, bool REC, int FORM>
+// PH: the phase-timing instantiation (run_trajectory).
+template <int C, int T, bool REC, int FORM, bool PH>
 __global__ void __launch_bounds__(64 * T) kmc_team_kernel(BatchParams p) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -41,8 +39,8 @@ __global__ void __launch_bounds__(64 * T) kmc_team_kernel(BatchParams p) {
       PosPolicy<C> pol;
       pol.init(pos, L, lane);
       for (int i = 0; i < gd.count; ++i)
-        run_trajectory<C, REC, PosPolicy<C>, T, true, STATE_CACHE>(p, gd.first + i, L, pol, lane,
-                                                                   team);
+        run_trajectory<C, REC, PosPolicy<C>, T, true, PH>(p, gd.first + i, L, pol,
+                                                                       lane, team);
     } else {
       // the waves' electrode factors (T*2*PMAX floats), the electrode weights,
       // the pair table and, for FORM 1, the c table
@@ -66,20 +64,24 @@ __global__ void __launch_bounds__(64 * T) kmc_team_kernel(BatchParams p) {
       __syncthreads();
       const TabPolicy<C, BT> pol = TabPolicy<C, BT>::make(tab, ctab, elw, evc, L.S, lane);
       for (int i = 0; i < gd.count; ++i)
-        run_trajectory<C, REC, TabPolicy<C, BT>, T, true, STATE_CACHE>(p, gd.first + i, L, pol,
-                                                                       lane, team);
+        run_trajectory<C, REC, TabPolicy<C, BT>, T, true, PH>(p, gd.first + i, L,
+                                                                           pol, lane, team);
     }
   }
 }
 
-#define KMC_TEAM(C, T)                                                  \
-  template __global__ void kmc_team_kernel<C, T, false, 1>(BatchParams); \
-  template __global__ void kmc_team_kernel<C, T, true, 1>(BatchParams);  \
-  template __global__ void kmc_team_kernel<C, T, false, 2>(BatchParams); \
-  template __global__ void kmc_team_kernel<C, T, true, 2>(BatchParams);  \
-  template __global__ void kmc_team_kernel<C, T, false, 3>(BatchParams); \
-  template __global__ void kmc_team_kernel<C, T, true, 3>(BatchParams);
+#define KMC_TEAM(C, T)                                                           \
+  template __global__ void kmc_team_kernel<C, T, false, 1, false>(BatchParams); \
+  template __global__ void kmc_team_kernel<C, T, true, 1, false>(BatchParams);  \
+  template __global__ void kmc_team_kernel<C, T, false, 2, false>(BatchParams); \
+  template __global__ void kmc_team_kernel<C, T, true, 2, false>(BatchParams);  \
+  template __global__ void kmc_team_kernel<C, T, false, 3, false>(BatchParams); \
+  template __global__ void kmc_team_kernel<C, T, true, 3, false>(BatchParams);
 KMC_TEAM(1, 2) KMC_TEAM(1, 4) KMC_TEAM(2, 2) KMC_TEAM(2, 4) KMC_TEAM(3, 2) KMC_TEAM(3, 4)
+#undef KMC_TEAM
+// phase timing: teams of 4 on two- and three-chunk networks
+#define KMC_TEAM(C, F) template __global__ void kmc_team_kernel<C, 4, false, F, true>(BatchParams);
+KMC_TEAM(2, 1) KMC_TEAM(2, 2) KMC_TEAM(2, 3) KMC_TEAM(3, 1) KMC_TEAM(3, 2) KMC_TEAM(3, 3)
 #undef KMC_TEAM
 
 // LDS bytes of a team workgroup: partial row sums plus the form's tables or

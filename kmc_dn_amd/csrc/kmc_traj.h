@@ -394,18 +394,63 @@ __device__ __forceinline__ float pick_lane_value(const float (&x)[C], int site) 
   return r;
 }
 
-// NC: entries of the trajectory's state cache (0: none).  A hop whose
-// occupation equals a cached one takes that state's row sums and FP64 prefix
-// from registers instead of evaluating them; the entries are emptied at every
-// FP64 recompute, so a trajectory's results depend on its
-// own history only, and ranges that start at multiples of the recompute
-// interval still continue exactly.  A new state enters at the front and the
-// oldest drops out.  The Go library keeps the same kind of
-// cache, keyed by occupation (simulation.go:254-296); at low bias a trajectory
-// mostly moves one hole back and forth between two sites, and two entries take
-// ~99 % of its hops.  In a team every wave keeps the same cache (the reduced
-// row sums are the same bits in every wave), so all waves hit together.
-template <int C, bool REC, class Pol, int T = 1, bool PRIO = true, int NC = 0>
+// One state of a trajectory's two-entry state cache: what a hop in that
+// state needs of it, all in registers.
+template <int C>
+struct StateEntry {
+  uint64_t key[C];   // occupation words
+  uint64_t nzr[C];   // lanes whose row sum is nonzero (in VGPRs: used on rare hops)
+  uint64_t pnz[C];   // lanes whose rate in row psel is nonzero
+  double tot;        // total rate (the FP64 prefix at the last lane), in a VGPR
+  double inc[C];     // FP64 inclusive prefix of the row sums over lanes
+  double exc[C];     // inc - row sum: the prefix before the lane
+  float se[C];       // FP32 site energies of the state
+  float pcum[C];     // FP32 prefix of row psel's rates over its partners, in site order
+  int psel;          // lane of that row (-1: none yet)
+  int udst;          // rows are sources, the walked set is the targets
+  int val;
+};
+
+// x in VGPRs: an inline-asm result with a VGPR operand counts as varying
+// across lanes, so x and what is computed from it stay in VGPRs.  For
+// wave-uniform values that only vector instructions (or rare paths) use,
+// keeping them out of the SGPRs that the hop loop runs short of.
+__device__ __forceinline__ double in_vgpr(double x) {
+  asm("" : "+v"(x));
+  return x;
+}
+__device__ __forceinline__ uint64_t in_vgpr(uint64_t x) {
+  asm("" : "+v"(x));
+  return x;
+}
+// A wave-uniform 64-bit value held in VGPRs, read into SGPRs.
+__device__ __forceinline__ uint64_t uniform_u64(uint64_t x) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)x);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(x >> 32));
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// The hop loop keeps a state cache of two entries: the current state and the
+// state before it.  Every hop changes the occupation, so the next state is
+// either the one before (a hit: nothing is evaluated, the cached row sums,
+// FP64 prefix, energies and, for the same selected row, the partner prefix
+// serve the hop) or new (a miss: its energies are the previous state's plus
+// the rank-1 update of the hop that left it, and its row sums are evaluated
+// into the older entry).  The entries swap roles on every hop; the loop is
+// unrolled by two hops, one for each assignment of the roles, so the entries
+// sit in fixed registers and a hit moves no data.  The entries are emptied at every FP64
+// recompute, so a trajectory's results depend on its own history only, and
+// ranges that start at multiples of the recompute interval continue exactly.
+// The Go library keeps a cache keyed by occupation too
+// (simulation.go:254-296); at low bias a trajectory mostly moves one hole
+// back and forth between two sites, and the two entries serve ~99 % of its
+// hops.  In a team every wave keeps the same cache (the reduced row sums are
+// the same bits in every wave), so all waves hit together.
+// PH: the phase-timing instantiation: shader-clock reads at the phase
+// boundaries of every hop (FP32_PHASES), summed into p.phases by the first wave
+// of each trajectory.  The reads wait for the wave's outstanding LDS and scalar
+// loads, so its hops are somewhat slower than the others'.
+template <int C, bool REC, class Pol, int T = 1, bool PRIO = true, bool PH = false>
 __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, const Pol& pol,
                                int lane, const Team<T, C>& team = Team<T, C>{}) {
 
@@ -503,46 +548,37 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
   if (h < h_stop && (h & 63) != 0) refill(h);
   // the next hop with work besides the hop itself: a multiple of 64 (uniforms),
   // the end of the prehops (tallies restart) or a multiple of K (recompute)
+  // (32-bit unsigned arithmetic: hh <= INT32_MAX, so r <= 2^31 and no 64-bit
+  // compare, which the scalar unit lacks, is needed)
   auto next_special = [&](int hh, bool inclusive) {
-    const int64_t r = inclusive ? ((int64_t)hh + 63) & ~(int64_t)63 : ((int64_t)(hh >> 6) + 1) << 6;
-    int n = (int)(r < INT32_MAX ? r : INT32_MAX);
+    const unsigned r = inclusive ? ((unsigned)hh + 63u) & ~63u : ((unsigned)hh | 63u) + 1u;
+    int n = (int)(r < (unsigned)INT32_MAX ? r : (unsigned)INT32_MAX);
     if (next_rc < n) n = next_rc;
     if ((inclusive ? prehops >= hh : prehops > hh) && prehops < n) n = prehops;
     return n;
   };
   int special = next_special(h, true);
   double b15 = 0.0, b31 = 0.0;   // row-broadcast registers of the FP64 scan
-  // state cache: keys (occupation words), validity, totals (wave-uniform) and
-  // the lanes' row sums and FP64 prefixes of each entry; a new state enters at
-  // entry 0 and the others move down one (the last drops out)
-  constexpr int NCE = NC > 0 ? NC : 1;
-  int nred = 0;   // team reductions so far
-  uint64_t ckey[NCE][C];
-  bool cval[NCE];
-  double ctot[NCE], cinc[NCE][C];
-  float cR[NCE][C];
-#pragma unroll
-  for (int e = 0; e < NCE; ++e) cval[e] = false;
-  for (; h < h_stop; ++h) {
-    if (__builtin_expect(h == special, 0)) {
-      if ((h & 63) == 0) refill(h);
-      if (h == prehops) {
-        time = 0.0;
-#pragma unroll
-        for (int c = 0; c < C; ++c) {
-          eo[c] = 0;
-          ot[c] = 0.0;
-        }
-      }
-      if (h == next_rc) {
-        recompute_energies<C>(se, occ, L, es, lane);
-        next_rc = next_multiple(h);
-#pragma unroll
-        for (int e = 0; e < NCE; ++e) cval[e] = false;
-      }
-      special = next_special(h, false);
+  uint64_t ph[FP32_PHASES] = {};
+  uint64_t tp = PH ? __builtin_amdgcn_s_memtime() : 0;
+  auto mark = [&](int k) {
+    if constexpr (PH) {
+      const uint64_t now = __builtin_amdgcn_s_memtime();
+      ph[k] += now - tp;
+      tp = now;
     }
+  };
+  int nred = 0;   // team reductions so far
+  StateEntry<C> E0, E1;
+  E0.val = E1.val = 0;
+  bool cur_ok = false;   // the current state's entry holds it
+  // the last hop's ends and whether its energy update is still to be applied
+  int psrc = 0, pdst = 0;
+  bool pend = false;
 
+  // The current state's row sums, FP64 prefix and total into W, from the
+  // energies se; false: no rate is positive and finite.
+  auto evaluate = [&](StateEntry<C>& W) __attribute__((always_inline)) -> bool {
     // roles of the lanes' sites, as wave-uniform lane masks (scalar ops only):
     // sources = occupied acceptors + electrodes, targets = empty acceptors + electrodes
     uint64_t srcm[C], dstm[C];
@@ -555,40 +591,9 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
       nocc += popcount64(o);
     }
     const bool udst = N - nocc <= nocc;   // walk the smaller set (empty vs occupied)
-    float R[C];
-    double incl[C];
-    double total = 0.0;
-    int hit = -1;
-    if constexpr (NC > 0) {
-#pragma unroll
-      for (int e = 0; e < NC; ++e) {
-        bool same = cval[e];
-#pragma unroll
-        for (int c = 0; c < C; ++c) same = same && ckey[e][c] == occ[c];
-        if (same) hit = e;
-      }
-    }
-    if (hit >= 0) {
-#pragma unroll
-      for (int c = 0; c < C; ++c) {
-        R[c] = cR[0][c];
-        incl[c] = cinc[0][c];
-      }
-      total = ctot[0];
-#pragma unroll
-      for (int e = 1; e < NCE; ++e)
-        if (hit == e) {
-#pragma unroll
-          for (int c = 0; c < C; ++c) {
-            R[c] = cR[e][c];
-            incl[c] = cinc[e][c];
-          }
-          total = ctot[e];
-        }
-      if constexpr (PRIO) __builtin_amdgcn_s_setprio(3);
-    } else {
     // every lane sums over the walked members; the sums of lanes outside the
     // other role (and past S) are discarded after the reduction
+    float R[C];
     uint64_t um[C];
 #pragma unroll
     for (int c = 0; c < C; ++c) {
@@ -636,58 +641,56 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
 #pragma unroll
     for (int c = 0; c < C; ++c)
       R[c] = __builtin_amdgcn_inverse_ballot_w64(udst ? srcm[c] : dstm[c]) ? R[c] : 0.f;
+    mark(1);
 
     // From here through the next hop's electrode terms the work is one
-    // dependency chain (scans, readlanes, the partner's and the Coulomb
-    // columns' LDS loads); the waves sharing a SIMD fill its gaps with their
-    // row sums of the walked members.  A raised issue priority lets the
-    // chain's instructions go as soon as they are ready: 3 from the scan to the
-    // partner, 1 for the update and the next hop's masks and electrode terms,
-    // 0 for the row sums (4096 trajectories: 12 % more hops per second).  PRIO
-    // false: the variant for launches whose groups queue behind full SIMDs.
+    // dependency chain (scans, readlanes, the partner row's LDS loads unless
+    // the state's entry holds its prefix, and a miss's Coulomb columns); the
+    // waves sharing a SIMD fill its gaps with their row sums of the walked
+    // members.  A raised issue priority lets the chain's instructions go as
+    // soon as they are ready: 3 from the scan to the partner, 1 for the update
+    // and the next hop's masks and electrode terms, 0 for the row sums (4096
+    // trajectories: 12 % more hops per second).  PRIO false: the variant for
+    // launches whose groups queue behind full SIMDs.
     if constexpr (PRIO) __builtin_amdgcn_s_setprio(3);
     // FP64 prefix over lane rows
+    double total = 0.0;
 #pragma unroll
     for (int c = 0; c < C; ++c) {
       if (c > 0 && 64 * c >= S) {
-        incl[c] = total;
+        W.inc[c] = W.exc[c] = total;
+        W.nzr[c] = in_vgpr((uint64_t)0);
         continue;
       }
-      incl[c] = c == 0 ? wave_scan_f64((double)R[c], b15, b31)
-                       : wave_scan_f64((double)R[c], b15, b31) + total;
-      total = readlane_f64(incl[c], 63);
+      W.inc[c] = c == 0 ? wave_scan_f64((double)R[c], b15, b31)
+                        : wave_scan_f64((double)R[c], b15, b31) + total;
+      W.exc[c] = W.inc[c] - (double)R[c];
+      W.nzr[c] = in_vgpr(ballot(R[c] > 0.f));
+      total = readlane_f64(W.inc[c], 63);
     }
-    if constexpr (NC > 0) {   // the new state enters at entry 0
 #pragma unroll
-      for (int e = NC - 1; e > 0; --e) {
-#pragma unroll
-        for (int c = 0; c < C; ++c) {
-          ckey[e][c] = ckey[e - 1][c];
-          cR[e][c] = cR[e - 1][c];
-          cinc[e][c] = cinc[e - 1][c];
-        }
-        ctot[e] = ctot[e - 1];
-        cval[e] = cval[e - 1];
-      }
-#pragma unroll
-      for (int c = 0; c < C; ++c) {
-        ckey[0][c] = occ[c];
-        cR[0][c] = R[c];
-        cinc[0][c] = incl[c];
-      }
-      ctot[0] = total;
-      cval[0] = true;
+    for (int c = 0; c < C; ++c) {
+      W.key[c] = occ[c];
+      W.se[c] = se[c];
     }
-    }
-    if (!(total > 0.0) || !(total < INFINITY)) {
-      status = ST_ZERO_RATE;
-      break;
-    }
+    W.tot = in_vgpr(total);
+    W.udst = udst;
+    W.psel = -1;
+    W.val = 1;
+    mark(2);
+    return total > 0.0 && total < INFINITY;
+  };
+
+  // One hop from the current state F (its entry valid); B: the other entry.
+  // Returns 1 when the state it leads to is B's, 0 when that state is new,
+  // -1 when the trajectory stops.
+  auto hop = [&](StateEntry<C>& F, const StateEntry<C>& B) __attribute__((always_inline)) -> int {
+    if constexpr (PRIO) __builtin_amdgcn_s_setprio(3);
     const float lg = readlane_f32(lg_lane, (int)(h & 63));
     // exponential waiting time -ln(1-u)/total, FP32 quotient into the FP64 clock
-    const double dt = (double)(lg * __builtin_amdgcn_rcpf((float)total));
+    const double dt = (double)(lg * __builtin_amdgcn_rcpf((float)F.tot));
     // event uniform x total, multiplied on every lane and read from this hop's
-    const double target = readlane_f64(ue_lane * total, (int)(h & 63));
+    const double target = readlane_f64(ue_lane * F.tot, (int)(h & 63));
 
     // the selected lane l and the remainder rr = target - (incl - R) at l, formed
     // on every lane (FP64 difference, rounded to FP32) and read from l
@@ -695,56 +698,73 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
     float rr = 0.f;
 #pragma unroll
     for (int c = 0; c < C; ++c) {
-      const uint64_t m = ballot(incl[c] > target);
+      const uint64_t m = ballot(F.inc[c] > target);
       if (sel < 0 && m) {
         const int l = __builtin_ctzll(m);
         sel = 64 * c + l;
-        rr = readlane_as_f32(target - (incl[c] - (double)R[c]), l);
+        rr = readlane_as_f32(target - F.exc[c], l);
       }
     }
     if (sel < 0) {   // target rounded onto the total: last lane with a nonzero row
 #pragma unroll
       for (int c = 0; c < C; ++c) {
-        const uint64_t m = ballot(R[c] > 0.f);
+        const uint64_t m = uniform_u64(F.nzr[c]);
         if (m) {
           const int l = 63 - __builtin_clzll(m);
           sel = 64 * c + l;
-          rr = readlane_as_f32(target - (incl[c] - (double)R[c]), l);
+          rr = readlane_as_f32(target - F.exc[c], l);
         }
       }
     }
+    mark(3);
 
-    // partner of `sel`: lane-parallel pass over its row, in site order
-    const float s_sel = pick_lane_value<C>(se, sel);
-    // sign of the lane-parallel difference: a scalar constant
-    const float sg = __int_as_float(udst ? (int)0xbf800000 : (int)0x3f800000);
-    const typename Pol::Pre pre = pol.prefetch(sel);
-    float run = 0.f;
+    // partner of `sel`: the prefix of its row over the lanes in site order (and
+    // the lanes whose rate is nonzero), from the entry when it was formed for
+    // the same row, else by a lane-parallel pass over the row
+    if (sel != F.psel) {
+      uint64_t inu[C];   // partners: targets of a source row, sources of a target row
+#pragma unroll
+      for (int c = 0; c < C; ++c) {
+        const uint64_t o = occ[c] & accm[c];
+        inu[c] = (F.udst ? accm[c] ^ o : o) | elm[c];
+      }
+      const float s_sel = pick_lane_value<C>(F.se, sel);
+      // sign of the lane-parallel difference: a scalar constant
+      const float sg = __int_as_float(F.udst ? (int)0xbf800000 : (int)0x3f800000);
+      const typename Pol::Pre pre = pol.prefetch(sel);
+      float run = 0.f;
+#pragma unroll
+      for (int c = 0; c < C; ++c) {
+        F.pcum[c] = 0.f;
+        F.pnz[c] = 0;
+        if (c > 0 && 64 * c >= S) continue;
+        // udst ? s_sel - se : se - s_sel (the sign is exact)
+        float r = fast_exp2(pol.pair(pre, c, F.se[c] - s_sel, sg));
+        r = __builtin_amdgcn_inverse_ballot_w64(inu[c]) ? r : 0.f;
+        F.pcum[c] = wave_scan_f32(r) + run;
+        F.pnz[c] = ballot(r > 0.f);
+        run = readlane_f32(F.pcum[c], 63);
+      }
+      F.psel = sel;
+    } else if constexpr (PH) {
+      ++ph[8];
+    }
     int pick = -1, last_nz = -1;
 #pragma unroll
     for (int c = 0; c < C; ++c) {
       if (c > 0 && 64 * c >= S) break;
-      const bool inU = __builtin_amdgcn_inverse_ballot_w64(udst ? dstm[c] : srcm[c]);
-      // udst ? s_sel - se : se - s_sel (the sign is exact)
-      float r = fast_exp2(pol.pair(pre, c, se[c] - s_sel, sg));
-      r = inU ? r : 0.f;
-      const float cum = wave_scan_f32(r) + run;
-      const uint64_t nz = ballot(r > 0.f);
-      const uint64_t hit = nz & ballot(cum > rr);
+      const uint64_t hit = F.pnz[c] & ballot(F.pcum[c] > rr);
       if (pick < 0) {
-        if (nz) last_nz = 64 * c + 63 - __builtin_clzll(nz);
+        if (F.pnz[c]) last_nz = 64 * c + 63 - __builtin_clzll(F.pnz[c]);
         if (hit) pick = 64 * c + __builtin_ctzll(hit);
       }
-      run = readlane_f32(cum, 63);
     }
     if (pick < 0) pick = last_nz;
-    if (pick < 0) {
-      status = ST_ZERO_RATE;
-      break;
-    }
+    if (pick < 0) return -1;
+    mark(4);
     if constexpr (PRIO) __builtin_amdgcn_s_setprio(1);
-    const int src = udst ? sel : pick;
-    const int dst = udst ? pick : sel;
+    const int src = F.udst ? sel : pick;
+    const int dst = F.udst ? pick : sel;
     const bool tally = h >= prehops;
 
     if (REC && tally && !p.post_hop_record) {
@@ -759,28 +779,17 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
       tr[dst * S + src] -= 1.0;
     }
     // perform hop: occupation (acceptors live in the words of the C chunks),
-    // rank-1 energy update (both Coulomb columns loaded together; an electrode
-    // end contributes 0), tallies
+    // tallies; the energy update is left to a miss
     const bool src_acc = src < N, dst_acc = dst < N;
-    float cs[C], cd[C];
-#pragma unroll
-    for (int c = 0; c < C; ++c) {
-      if constexpr (Pol::kZeroColumnN) {   // column N holds c = 0: no select per lane
-        cs[c] = (c == 0 || 64 * c < S) ? pol.coul(min((unsigned)src, (unsigned)N), c) : 0.f;
-        cd[c] = (c == 0 || 64 * c < S) ? pol.coul(min((unsigned)dst, (unsigned)N), c) : 0.f;
-      } else {
-        cs[c] = (c == 0 || 64 * c < S) && src_acc ? pol.coul(src, c) : 0.f;
-        cd[c] = (c == 0 || 64 * c < S) && dst_acc ? pol.coul(dst, c) : 0.f;
-      }
-    }
 #pragma unroll
     for (int c = 0; c < C; ++c) {
       // the source's bit is cleared whether it is an acceptor or not: bits of
       // electrode sites are never set (a one-chunk kernel has S <= 64 sites)
       if (C == 1 || (src >> 6) == c) occ[c] &= ~(1ull << (src & 63));
       if (dst_acc && (C == 1 || (dst >> 6) == c)) occ[c] |= 1ull << (dst & 63);
-      se[c] = (se[c] - cs[c]) + cd[c];
     }
+    psrc = src;
+    pdst = dst;
     // electrode tallies (a wave-uniform branch: most hops join two acceptors);
     // they move on the lanes of both ends; only electrode lanes' are kept
     if (__builtin_expect(!src_acc || !dst_acc, 0)) {
@@ -796,10 +805,88 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
         if (__builtin_amdgcn_inverse_ballot_w64(occ[c] & accm[c])) ot[c] += dt;
     }
     time += dt;
-  }
+    mark(5);
+    bool same = B.val != 0;
+#pragma unroll
+    for (int c = 0; c < C; ++c) same = same && B.key[c] == occ[c];
+    return same ? 1 : 0;
+  };
+
+  // One hop whose current state's entry is F (B: the entry of the state
+  // before); false: the trajectory stops here or its range ends.
+  auto step = [&](StateEntry<C>& F, StateEntry<C>& B) __attribute__((always_inline)) -> bool {
+    if (__builtin_expect(h == special, 0)) {
+      if ((h & 63) == 0) refill(h);
+      if (h == prehops) {
+        time = 0.0;
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+          eo[c] = 0;
+          ot[c] = 0.0;
+        }
+      }
+      if (h == next_rc) {
+        recompute_energies<C>(se, occ, L, es, lane);
+        next_rc = next_multiple(h);
+        pend = false;
+        cur_ok = false;
+        E0.val = E1.val = 0;
+      }
+      special = next_special(h, false);
+    }
+    mark(0);
+    if (!cur_ok) {
+      // a new state: its energies are the previous state's plus the last hop's
+      // rank-1 update (both Coulomb columns loaded together; an electrode end
+      // contributes 0), or the recomputed ones
+      if (pend) {
+        const bool src_acc = psrc < N, dst_acc = pdst < N;
+        float cs[C], cd[C];
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+          if constexpr (Pol::kZeroColumnN) {   // column N holds c = 0: no select per lane
+            cs[c] = (c == 0 || 64 * c < S) ? pol.coul(min((unsigned)psrc, (unsigned)N), c) : 0.f;
+            cd[c] = (c == 0 || 64 * c < S) ? pol.coul(min((unsigned)pdst, (unsigned)N), c) : 0.f;
+          } else {
+            cs[c] = (c == 0 || 64 * c < S) && src_acc ? pol.coul(psrc, c) : 0.f;
+            cd[c] = (c == 0 || 64 * c < S) && dst_acc ? pol.coul(pdst, c) : 0.f;
+          }
+        }
+#pragma unroll
+        for (int c = 0; c < C; ++c) se[c] = (B.se[c] - cs[c]) + cd[c];
+      }
+      if (!evaluate(F)) {
+        status = ST_ZERO_RATE;
+        return false;
+      }
+      cur_ok = true;
+    } else if constexpr (PH) {
+      ++ph[7];
+    }
+    const int r = hop(F, B);
+    if (r < 0) {
+      status = ST_ZERO_RATE;
+      return false;
+    }
+    cur_ok = r > 0;
+    pend = true;
+    ++h;
+    mark(6);
+    return h < h_stop;
+  };
+  if (h < h_stop)
+    for (;;) {
+      if (!step(E0, E1)) break;
+      if (!step(E1, E0)) break;
+    }
 
   if constexpr (PRIO) __builtin_amdgcn_s_setprio(0);   // a stopped trajectory leaves the loop raised
   if (!team.writer()) return;
+  if constexpr (PH) {
+    if (p.phases && lane == 0)
+      for (int k = 0; k < FP32_PHASES; ++k)
+        atomicAdd(reinterpret_cast<unsigned long long*>(p.phases + k), (unsigned long long)ph[k]);
+  }
   if (lane == 0) {
     const int64_t dt_ticks = wall_clock64() - t_start;
     o->ticks = (int32_t)(dt_ticks < INT32_MAX ? dt_ticks : INT32_MAX);

@@ -122,7 +122,19 @@ struct BatchParams {
   int64_t traffic_stride;       // >= S*S of every layout in the batch
   int32_t post_hop_record;      // 1: numba convention (post-hop state)
   int32_t resume;               // 1: run [h_begin, h_end) from the state in `out`
+  uint64_t* phases;             // [FP32_PHASES] sums over trajectories (the phase-timing
+                                // instantiations only), or null
 };
+
+// Hop phases of the FP32 batched kernels timed by their phase-timing
+// instantiations (shader cycles of each trajectory's first wave, summed over
+// trajectories): 0 uniforms and FP64 recomputes, 1 state-cache lookup, energy
+// update, electrode terms, row sums and team reduction of a miss, 2 FP64 prefix scan and cache
+// insert of a miss (with the energy update the last hop left pending), 3
+// clock and event selection, 4 partner selection, 5 hop (occupation, tallies),
+// 6 lookup and copy of a hit; slot 7 counts the hits, slot 8 the hops whose
+// partner prefix came from the state's entry.
+constexpr int FP32_PHASES = 9;
 
 struct PlainParams {
   const TrajDev* trajs;

@@ -7,6 +7,8 @@ lds:  LDS read bandwidth of strided ds_read_b32/b64/b128 (--width 4/8/16) and of
 lone: the hop latency of one trajectory alone (the long tail of an uneven batch
       or of a served request) on a network of N acceptors and P electrodes;
 fp64phases: shader cycles per hop and phase of the FP64 dense kernel;
+fp32phases: the same for the FP32 batched kernels, on the 4096-trajectory N=30
+      ensemble and on lone trajectories (solo and team);
 chain: latency of dependent FP64 / FP32 adds in one wave.
 """
 from __future__ import annotations
@@ -92,6 +94,77 @@ def fp64_phases(hops: int = 20_000, seed: int = 0) -> dict:
             "cycles_per_hop_total": round(sum(per), 1)}
 
 
+FP32_PHASES = ["uniforms_recompute", "miss_update_row_sums", "miss_scan_insert", "clock_select",
+               "partner", "hop", "hit_copy"]
+
+
+def _fp32_phase_run(b, run: dict, hops_total: int) -> dict:
+    """Phase cycles per hop of batch b (run options `run`) beside the ordinary
+    kernel's HIP-event time."""
+    for _ in range(2):
+        b.run(**run)
+        ms = b.sync()
+    o = _lib.kmc_run_opts(_lib.FP32_BATCHED, int(run.get("waves_per_group", 0)), 0, 0, 0, 0, 0,
+                          int(run.get("table_form", 0)), 0, int(run.get("team", 1)))
+    cyc = (C.c_uint64 * 9)()
+    _lib.check(_lib.load().kmc_probe_fp32_phases(b.id, C.byref(o), cyc, 9))
+    per = [cyc[k] / hops_total for k in range(len(FP32_PHASES))]
+    return {"kernel_ms": round(ms, 3), "hit_rate": round(cyc[7] / hops_total, 5),
+            "partner_hit_rate": round(cyc[8] / hops_total, 5),
+            "cycles_per_hop": dict(zip(FP32_PHASES, [round(x, 1) for x in per])),
+            "cycles_per_hop_total": round(sum(per), 1)}
+
+
+def fp32_phases_ensemble(K: int = 4096, hops: int = 100_000) -> dict:
+    """Phase cycles per hop and trajectory of the solo table kernel (4 waves
+    per workgroup, {lt, lt+c} table) over the K-trajectory N=30 ensemble: every
+    wave's cycles, with the waves that share its SIMD interleaved."""
+    import numpy as np
+    from . import workloads
+    ens = workloads.ensemble(0, K, hops)
+    net = ens.net
+    lay = engine.Layout(net.N, net.P, net.nu, net.kT, net.I_0, net.R, net.distances,
+                        net.transitions_constant)
+    try:
+        es = lay.energy_sets(net.E_constant, net.site_energies[net.N:])
+        b = engine.Batch(engine.make_trajs(K, lay.id, es[0], ens.seeds, hops, net.occupation))
+        try:
+            r = _fp32_phase_run(b, dict(precision=_lib.FP32_BATCHED, waves_per_group=4,
+                                        table_form=1), K * hops)
+        finally:
+            b.close()
+    finally:
+        lay.close()
+    r.update({"workload": f"{K} x {hops} hops, N=30 P=8", "waves_per_group": 4, "table_form": 1})
+    return r
+
+
+def fp32_phases_lone(N: int, P: int, form: int, team: int, hops: int = 100_000,
+                     seed: int = 1) -> dict:
+    """Phase cycles per hop of one trajectory alone on workloads.network_3d(seed,
+    N, P): solo (4-wave workgroup, one trajectory) or a team of 4."""
+    import numpy as np
+    from . import workloads
+    net = workloads.network_3d(seed, N, P)
+    lay = engine.Layout(N, P, net.nu, net.kT, net.I_0, net.R, net.distances,
+                        net.transitions_constant)
+    try:
+        lay.set_positions(net.ab, np.concatenate([net.acceptors, net.electrodes[:, :3]], 0))
+        es = lay.energy_sets(net.E_constant, net.site_energies[N:])
+        b = engine.Batch(engine.make_trajs(1, lay.id, es[0], np.arange(1, dtype=np.uint64), hops,
+                                           net.occupation))
+        try:
+            r = _fp32_phase_run(b, dict(precision=_lib.FP32_BATCHED, table_form=form, team=team,
+                                        waves_per_group=4 if team <= 1 else 1), hops)
+        finally:
+            b.close()
+    finally:
+        lay.close()
+    r.update({"S": N + P, "N": N, "P": P, "form": form, "team": team, "hops": hops,
+              "us_per_hop": round(r["kernel_ms"] * 1e3 / hops, 3)})
+    return r
+
+
 CHAIN_KINDS = ["v_add_f64 dependent (64 lanes)", "v_add_f64 dependent (1 lane)",
                "v_add_f32 dependent", "v_fma_f64 dependent", "v_add_f64 4 interleaved chains"]
 
@@ -109,7 +182,7 @@ def chain_latency(iters: int = 4096) -> dict:
 
 def main() -> None:
     ap = argparse.ArgumentParser()
-    ap.add_argument("what", choices=["copy", "lds", "lone", "fp64phases", "chain"])
+    ap.add_argument("what", choices=["copy", "lds", "lone", "fp64phases", "fp32phases", "chain"])
     ap.add_argument("--bytes", type=int, default=1 << 30)
     ap.add_argument("--repeats", type=int, default=5)
     ap.add_argument("--width", type=int, default=-1, help="4, 8, 16, 0 (stream); -1 all")
@@ -126,6 +199,13 @@ def main() -> None:
         print(json.dumps(lone(args.N, args.P, args.form, args.team, args.hops)))
     elif args.what == "fp64phases":
         print(json.dumps(fp64_phases(args.hops)))
+    elif args.what == "fp32phases":
+        print(json.dumps(fp32_phases_ensemble()))
+        for N, P, form in [(30, 8, 1), (56, 8, 1), (120, 12, 2), (120, 12, 3)]:
+            for team in (1, 4):
+                if team == 1 and form == 3 or team == 4 and N + P <= 64:
+                    continue
+                print(json.dumps(fp32_phases_lone(N, P, form, team)), flush=True)
     elif args.what == "chain":
         print(json.dumps(chain_latency()))
     else:
