@@ -402,11 +402,12 @@ struct StateEntry {
   uint64_t nzr[C];   // lanes whose row sum is nonzero (in VGPRs: used on rare hops)
   uint64_t pnz[C];   // lanes whose rate in row psel is nonzero
   double tot;        // total rate (the FP64 prefix at the last lane), in a VGPR
+  float rtot;        // 1 / (float)tot (v_rcp_f32), the clock's rate factor, in a VGPR
   double inc[C];     // FP64 inclusive prefix of the row sums over lanes
   double exc[C];     // inc - row sum: the prefix before the lane
   float se[C];       // FP32 site energies of the state
   float pcum[C];     // FP32 prefix of row psel's rates over its partners, in site order
-  int psel;          // lane of that row (-1: none yet)
+  int psel;          // lane of that row (-2: none yet; never a lane or the -1 of no lane)
   int udst;          // rows are sources, the walked set is the targets
   int val;
 };
@@ -558,6 +559,8 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
     return n;
   };
   int special = next_special(h, true);
+  // hops below h_lim need nothing besides the hop itself (the fast path's range)
+  int h_lim = special < h_stop ? special : h_stop;
   double b15 = 0.0, b31 = 0.0;   // row-broadcast registers of the FP64 scan
   uint64_t ph[FP32_PHASES] = {};
   uint64_t tp = PH ? __builtin_amdgcn_s_memtime() : 0;
@@ -674,8 +677,9 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
       W.se[c] = se[c];
     }
     W.tot = in_vgpr(total);
+    W.rtot = __builtin_amdgcn_rcpf((float)total);
     W.udst = udst;
-    W.psel = -1;
+    W.psel = -2;
     W.val = 1;
     mark(2);
     return total > 0.0 && total < INFINITY;
@@ -688,7 +692,7 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
     if constexpr (PRIO) __builtin_amdgcn_s_setprio(3);
     const float lg = readlane_f32(lg_lane, (int)(h & 63));
     // exponential waiting time -ln(1-u)/total, FP32 quotient into the FP64 clock
-    const double dt = (double)(lg * __builtin_amdgcn_rcpf((float)F.tot));
+    const double dt = (double)(lg * F.rtot);
     // event uniform x total, multiplied on every lane and read from this hop's
     const double target = readlane_f64(ue_lane * F.tot, (int)(h & 63));
 
@@ -833,6 +837,7 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
         E0.val = E1.val = 0;
       }
       special = next_special(h, false);
+      h_lim = special < h_stop ? special : h_stop;
     }
     mark(0);
     if (!cur_ok) {
@@ -871,14 +876,118 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
     cur_ok = r > 0;
     pend = true;
     ++h;
-    mark(6);
+    mark(5);
     return h < h_stop;
   };
-  if (h < h_stop)
-    for (;;) {
+  // A hit hop in its shortest form: the current state's entry F is valid, its
+  // partner prefix was formed for the row the event falls in, both ends are
+  // acceptors and the hop leads back into the state of the other entry B.  The
+  // arithmetic is the general hop's (same target, remainder, partner and clock
+  // bits); only the bookkeeping that such a hop cannot need is left out, so the
+  // scalar unit that the waves of a CU share issues a third of the general
+  // path's instructions.  Anything else (an uncached row, an electrode end, a
+  // new state, the end of the range, uniforms or a recompute due) returns false
+  // with nothing changed, and the general step takes the hop.  Kernels that
+  // record take the general step on every hop.
+  auto fast_hop = [&](const StateEntry<C>& F, const StateEntry<C>& B)
+                      __attribute__((always_inline)) -> bool {
+    if constexpr (REC) return false;
+    if (h >= h_lim) return false;
+    const int hl = h & 63;
+    const double target = readlane_f64(ue_lane * F.tot, hl);
+    // sel and pick are -1 when no lane qualifies (s_ff1 of an empty mask); a
+    // readlane from lane -1 reads lane 63 and its value goes unused: psel is
+    // never -1, and -1 as unsigned is past every acceptor
+    int sel = -1;
+    float rr = 0.f;
+    if constexpr (C == 1) {
+      sel = lowest_bit(ballot(F.inc[0] > target));
+      rr = readlane_as_f32(target - F.exc[0], sel);
+    } else {
+#pragma unroll
+      for (int c = 0; c < C; ++c) {
+        if (c > 0 && 64 * c >= S) break;
+        const uint64_t m = ballot(F.inc[c] > target);
+        if (sel < 0 && m) {
+          const int l = __builtin_ctzll(m);
+          sel = 64 * c + l;
+          rr = readlane_as_f32(target - F.exc[c], l);
+        }
+      }
+    }
+    if (sel != F.psel) return false;
+    int pick = -1;
+    if constexpr (C == 1) {
+      pick = lowest_bit(F.pnz[0] & ballot(F.pcum[0] > rr));
+    } else {
+#pragma unroll
+      for (int c = 0; c < C; ++c) {
+        if (c > 0 && 64 * c >= S) break;
+        const uint64_t hit = F.pnz[c] & ballot(F.pcum[c] > rr);
+        if (pick < 0 && hit) pick = 64 * c + __builtin_ctzll(hit);
+      }
+    }
+    if (pick < 0) return false;   // no partner: the general step's fallback
+    // one end occupied (or an electrode), the other empty (or an electrode):
+    // flipping both bits and clearing the electrode sites' bits is the hop
+    // whichever end is the source
+    uint64_t nocc[C];
+    bool back = true;
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      nocc[c] = occ[c];
+      if (C == 1 || (sel >> 6) == c) nocc[c] ^= 1ull << (sel & 63);
+      if (C == 1 || (pick >> 6) == c) nocc[c] ^= 1ull << (pick & 63);
+      nocc[c] &= accm[c];
+      back = back && nocc[c] == B.key[c];
+    }
+    if (!back) return false;
+#pragma unroll
+    for (int c = 0; c < C; ++c) occ[c] = nocc[c];
+    // electrode tallies on every lane (only the electrode lanes' are kept): the
+    // rows are the sources when the walked set is the targets
+    const int src = F.udst ? sel : pick, dst = F.udst ? pick : sel;
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      const int s = lane + 64 * c;
+      eo[c] += (s == dst ? 1 : 0) - (s == src ? 1 : 0);
+    }
+    time += (double)(readlane_f32(lg_lane, hl) * F.rtot);
+    ++h;
+    if constexpr (PH) {
+      ++ph[7];
+      ++ph[8];
+    }
+    mark(6);
+    return true;
+  };
+
+  // Hops alternate between the entries.  `one`: entry E1 holds (or, after a
+  // miss, is to hold) the current state.  A general step, then fast hops for
+  // as long as they apply (the current state's entry must hold it).
+  bool one = false;
+  while (h < h_stop) {
+    if (!one) {
       if (!step(E0, E1)) break;
+    } else {
       if (!step(E1, E0)) break;
     }
+    one = !one;
+    if (!cur_ok) continue;
+    if (one) {
+      while (fast_hop(E1, E0))
+        if (!fast_hop(E0, E1)) {
+          one = false;
+          break;
+        }
+    } else {
+      while (fast_hop(E0, E1))
+        if (!fast_hop(E1, E0)) {
+          one = true;
+          break;
+        }
+    }
+  }
 
   if constexpr (PRIO) __builtin_amdgcn_s_setprio(0);   // a stopped trajectory leaves the loop raised
   if (!team.writer()) return;
