@@ -175,6 +175,41 @@ __device__ __forceinline__ uint32_t wave_max_u32(uint32_t x) {
   return x;
 }
 
+// Maximum over the 64 lanes (every lane active), the same on every lane: the
+// row shifts and row broadcasts of the scans with max; a lane with no source
+// lane keeps its own value (bound_ctrl off, `old` = x), which max leaves alone.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_keep_f64(double x) {
+  const uint64_t b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_update_dpp((int)(uint32_t)b, (int)(uint32_t)b, CTRL, ROW_MASK, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp((int)(uint32_t)(b >> 32), (int)(uint32_t)(b >> 32), CTRL,
+                                             ROW_MASK, 0xF, false);
+  return __longlong_as_double((long long)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo));
+}
+__device__ __forceinline__ double wave_max_f64(double x) {
+  x = fmax(x, dpp_keep_f64<DPP_ROW_SHR1, 0xF>(x));
+  x = fmax(x, dpp_keep_f64<DPP_ROW_SHR2, 0xF>(x));
+  x = fmax(x, dpp_keep_f64<DPP_ROW_SHR4, 0xF>(x));
+  x = fmax(x, dpp_keep_f64<DPP_ROW_SHR8, 0xF>(x));
+  x = fmax(x, dpp_keep_f64<DPP_ROW_BCAST15, 0xA>(x));
+  x = fmax(x, dpp_keep_f64<DPP_ROW_BCAST31, 0xC>(x));
+  return readlane_f64(x, 63);
+}
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ float dpp_keep_f32(float x) {
+  return __int_as_float(
+      __builtin_amdgcn_update_dpp(__float_as_int(x), __float_as_int(x), CTRL, ROW_MASK, 0xF, false));
+}
+__device__ __forceinline__ float wave_max_f32(float x) {
+  x = fmaxf(x, dpp_keep_f32<DPP_ROW_SHR1, 0xF>(x));
+  x = fmaxf(x, dpp_keep_f32<DPP_ROW_SHR2, 0xF>(x));
+  x = fmaxf(x, dpp_keep_f32<DPP_ROW_SHR4, 0xF>(x));
+  x = fmaxf(x, dpp_keep_f32<DPP_ROW_SHR8, 0xF>(x));
+  x = fmaxf(x, dpp_keep_f32<DPP_ROW_BCAST15, 0xA>(x));
+  x = fmaxf(x, dpp_keep_f32<DPP_ROW_BCAST31, 0xC>(x));
+  return readlane_f32(x, 63);
+}
+
 __device__ __forceinline__ float fast_exp2(float x) { return __builtin_amdgcn_exp2f(x); }
 __device__ __forceinline__ float fast_log2(float x) { return __builtin_amdgcn_logf(x); }
 

@@ -410,6 +410,13 @@ struct StateEntry {
   int psel;          // lane of that row (-2: none yet; never a lane or the -1 of no lane)
   int udst;          // rows are sources, the walked set is the targets
   int val;
+  // The hop from this state back into the other entry's state, as the leaps
+  // of one-chunk kernels test it (uniform values held in VGPRs):
+  int plast;         // its partner lane, in row psel (-1: unknown since either entry changed)
+  double lo, hi;     // row psel is the selected row iff lo <= target < hi
+  double exs;        // exc of row psel
+  float plo, phi;    // lane plast is the selected partner iff plo <= rr < phi
+  int dlt;           // per lane: its change of the tallies (+1 destination, -1 source)
 };
 
 // x in VGPRs: an inline-asm result with a VGPR operand counts as varying
@@ -421,6 +428,10 @@ __device__ __forceinline__ double in_vgpr(double x) {
   return x;
 }
 __device__ __forceinline__ uint64_t in_vgpr(uint64_t x) {
+  asm("" : "+v"(x));
+  return x;
+}
+__device__ __forceinline__ float in_vgpr(float x) {
   asm("" : "+v"(x));
   return x;
 }
@@ -574,6 +585,7 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
   int nred = 0;   // team reductions so far
   StateEntry<C> E0, E1;
   E0.val = E1.val = 0;
+  E0.plast = E1.plast = -1;
   bool cur_ok = false;   // the current state's entry holds it
   // the last hop's ends and whether its energy update is still to be applied
   int psrc = 0, pdst = 0;
@@ -680,6 +692,7 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
     W.rtot = __builtin_amdgcn_rcpf((float)total);
     W.udst = udst;
     W.psel = -2;
+    W.plast = -1;
     W.val = 1;
     mark(2);
     return total > 0.0 && total < INFINITY;
@@ -750,6 +763,7 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
         run = readlane_f32(F.pcum[c], 63);
       }
       F.psel = sel;
+      F.plast = -1;
     } else if constexpr (PH) {
       ++ph[8];
     }
@@ -860,6 +874,7 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
 #pragma unroll
         for (int c = 0; c < C; ++c) se[c] = (B.se[c] - cs[c]) + cd[c];
       }
+      B.plast = -1;   // B's hop back led into the state that F held
       if (!evaluate(F)) {
         status = ST_ZERO_RATE;
         return false;
@@ -880,24 +895,67 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
     return h < h_stop;
   };
   // A hit hop in its shortest form: the current state's entry F is valid, its
-  // partner prefix was formed for the row the event falls in, both ends are
-  // acceptors and the hop leads back into the state of the other entry B.  The
-  // arithmetic is the general hop's (same target, remainder, partner and clock
-  // bits); only the bookkeeping that such a hop cannot need is left out, so the
-  // scalar unit that the waves of a CU share issues a third of the general
-  // path's instructions.  Anything else (an uncached row, an electrode end, a
-  // new state, the end of the range, uniforms or a recompute due) returns false
-  // with nothing changed, and the general step takes the hop.  Kernels that
-  // record take the general step on every hop.
-  auto fast_hop = [&](const StateEntry<C>& F, const StateEntry<C>& B)
+  // partner prefix was formed for the row the event falls in and the hop leads
+  // back into the state of the other entry B.  The arithmetic is the general
+  // hop's (same target, remainder, partner and clock bits); only the
+  // bookkeeping that such a hop cannot need is left out, so the scalar unit
+  // that the waves of a CU share issues a third of the general path's
+  // instructions.  Anything else (an uncached row, a new state, the end of the
+  // range, uniforms or a recompute due) returns false with nothing changed, and
+  // the general step takes the hop.  Kernels that record take the general step
+  // on every hop.
+  //
+  // Leaps (one-chunk kernels): once the hop back is known from both states
+  // (plast), whether hop k of the current uniform block takes it depends only
+  // on its own uniform, so the lanes test the hops of the block in parallel,
+  // lane k for hop (h & ~63) + k in the state that hop starts from (F and B
+  // alternate): target in [lo, hi) selects the row, rr in [plo, phi) the
+  // partner, with the general hop's operations.  The run of consecutive hops
+  // back, cut to an odd length (the state then is B's), is applied at once: the
+  // clock adds the hops' waiting times one by one in hop order, the tallies
+  // change by the count of hops each way, the occupation is B's.  The bits are
+  // those of taking the hops one at a time.
+  auto fast_hop = [&](StateEntry<C>& F, const StateEntry<C>& B)
                       __attribute__((always_inline)) -> bool {
     if constexpr (REC) return false;
     if (h >= h_lim) return false;
     const int hl = h & 63;
-    const double target = readlane_f64(ue_lane * F.tot, hl);
+    const double tlane = ue_lane * F.tot;   // every lane's hop's target in state F
+    if constexpr (C == 1) {
+      if ((F.plast | B.plast) >= 0) {
+        const double tb = ue_lane * B.tot;
+        const float rf = (float)(tlane - F.exs), rb = (float)(tb - B.exs);
+        const uint64_t okf =
+            ballot(tlane >= F.lo && tlane < F.hi && rf >= F.plo && rf < F.phi);
+        const uint64_t okb = ballot(tb >= B.lo && tb < B.hi && rb >= B.plo && rb < B.phi);
+        // lanes k with k - hl even start their hop in state F
+        const uint64_t inf = (hl & 1) ? 0xAAAAAAAAAAAAAAAAull : 0x5555555555555555ull;
+        const uint64_t run = ((okf & inf) | (okb & ~inf)) >> hl;
+        // trailing ones (s_ff1 of an all-ones complement is -1: the whole block)
+        int n = lowest_bit(~run);
+        const int nmax = 64 - hl < h_lim - h ? 64 - hl : h_lim - h;
+        if ((unsigned)n > (unsigned)nmax) n = nmax;
+        if (n == 0) return false;
+        n -= (n & 1) ^ 1;
+        const double dtl = (double)(lg_lane * (__builtin_amdgcn_inverse_ballot_w64(inf)
+                                                   ? F.rtot : B.rtot));
+        for (int k = hl; k < hl + n; ++k) time += readlane_f64(dtl, k);
+        eo[0] += ((n + 1) >> 1) * F.dlt + (n >> 1) * B.dlt;
+        occ[0] = B.key[0];
+        h += n;
+        if constexpr (PH) {
+          ph[7] += n;
+          ph[8] += n;
+          ph[9] += n;
+        }
+        mark(6);
+        return true;
+      }
+    }
+    const double target = readlane_f64(tlane, hl);
     // sel and pick are -1 when no lane qualifies (s_ff1 of an empty mask); a
     // readlane from lane -1 reads lane 63 and its value goes unused: psel is
-    // never -1, and -1 as unsigned is past every acceptor
+    // never -1
     int sel = -1;
     float rr = 0.f;
     if constexpr (C == 1) {
@@ -954,6 +1012,19 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
     }
     time += (double)(readlane_f32(lg_lane, hl) * F.rtot);
     ++h;
+    if constexpr (C == 1) {
+      if (F.plast < 0) {
+        // the hop back from F is known: its row's and partner's bounds
+        F.plast = pick;
+        F.hi = in_vgpr(readlane_f64(F.inc[0], sel));
+        F.exs = in_vgpr(readlane_f64(F.exc[0], sel));
+        F.lo = in_vgpr(wave_max_f64(lane < sel ? F.inc[0] : -INFINITY));
+        F.phi = in_vgpr(readlane_f32(F.pcum[0], pick));
+        F.plo = in_vgpr(wave_max_f32(lane < pick && ((F.pnz[0] >> lane) & 1ull)
+                                         ? F.pcum[0] : -INFINITY));
+        F.dlt = (lane == dst ? 1 : 0) - (lane == src ? 1 : 0);
+      }
+    }
     if constexpr (PH) {
       ++ph[7];
       ++ph[8];

@@ -128,13 +128,14 @@ struct BatchParams {
 
 // Hop phases of the FP32 batched kernels timed by their phase-timing
 // instantiations (shader cycles of each trajectory's first wave, summed over
-// trajectories): 0 uniforms and FP64 recomputes, 1 state-cache lookup, energy
-// update, electrode terms, row sums and team reduction of a miss, 2 FP64 prefix scan and cache
-// insert of a miss (with the energy update the last hop left pending), 3
-// clock and event selection, 4 partner selection, 5 hop (occupation, tallies),
-// 6 lookup and copy of a hit; slot 7 counts the hits, slot 8 the hops whose
-// partner prefix came from the state's entry.
-constexpr int FP32_PHASES = 9;
+// trajectories): 0 uniforms and FP64 recomputes (and fast-hop attempts that
+// fell through to the general step), 1 energy update, electrode terms, row
+// sums and team reduction of a miss, 2 FP64 prefix scan and cache insert of a
+// miss, 3 clock and event selection, 4 partner selection, 5 hop (occupation,
+// tallies) of the general step, 6 fast hops and leaps; slot 7 counts the hops
+// from a cached state, slot 8 those whose partner prefix came from the state's
+// entry, slot 9 those taken by leaps.
+constexpr int FP32_PHASES = 10;
 
 struct PlainParams {
   const TrajDev* trajs;

@@ -95,7 +95,7 @@ def fp64_phases(hops: int = 20_000, seed: int = 0) -> dict:
 
 
 FP32_PHASES = ["uniforms_recompute", "miss_update_row_sums", "miss_scan_insert", "clock_select",
-               "partner", "hop", "hit_copy"]
+               "partner", "hop", "fast_hops_leaps"]
 
 
 def _fp32_phase_run(b, run: dict, hops_total: int) -> dict:
@@ -106,11 +106,12 @@ def _fp32_phase_run(b, run: dict, hops_total: int) -> dict:
         ms = b.sync()
     o = _lib.kmc_run_opts(_lib.FP32_BATCHED, int(run.get("waves_per_group", 0)), 0, 0, 0, 0, 0,
                           int(run.get("table_form", 0)), 0, int(run.get("team", 1)))
-    cyc = (C.c_uint64 * 9)()
-    _lib.check(_lib.load().kmc_probe_fp32_phases(b.id, C.byref(o), cyc, 9))
+    cyc = (C.c_uint64 * 10)()
+    _lib.check(_lib.load().kmc_probe_fp32_phases(b.id, C.byref(o), cyc, 10))
     per = [cyc[k] / hops_total for k in range(len(FP32_PHASES))]
     return {"kernel_ms": round(ms, 3), "hit_rate": round(cyc[7] / hops_total, 5),
             "partner_hit_rate": round(cyc[8] / hops_total, 5),
+            "leap_rate": round(cyc[9] / hops_total, 5),
             "cycles_per_hop": dict(zip(FP32_PHASES, [round(x, 1) for x in per])),
             "cycles_per_hop_total": round(sum(per), 1)}
 
