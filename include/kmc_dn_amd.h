@@ -205,6 +205,14 @@ typedef struct kmc_run_opts {
   int32_t team;               /* FP32: waves per trajectory, 1 (0) solo, 2 or 4: the
                                  walked set of each hop is split over the team (lower
                                  hop latency for batches that do not fill the GPU) */
+  int32_t time_limit_us;      /* FP32 solo kernels: a trajectory that has run this many
+                                 microseconds of device time stops at its next
+                                 recompute point (a multiple of recompute_interval)
+                                 with status KMC_OK and hops_done short of its range;
+                                 a resume run from there continues bit-exactly.  Bounds
+                                 the duration of a launch (schedulers, watchdogs).
+                                 0 = none.  Refused with team kernels.  No reference
+                                 counterpart. */
 } kmc_run_opts;
 
 /* Uploads trajectory descriptors and allocates device outputs. */

@@ -53,6 +53,14 @@ TEAM_MIN_GAIN = 1.2
 # one-chunk team entry 1.89 s.
 TEAM_SIZE = 4
 TEAM_MIN_EDGE = 128
+# Latency probe of the team thresholds (probe_team_thresholds): warm-up hops,
+# timed hops, and the ratios measured on the 2048-network batch: a team's hop
+# latency over the probed one (S = 81: 1.05 against 2.26 us/hop), a solo
+# trajectory's inside the batch over the probed one
+PROBE_WARM = 4096
+PROBE_HOPS = 16384
+PROBE_TEAM = 0.5
+PROBE_SOLO = 1.5
 CACHE_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "autotune_cache.json")
 
 
@@ -214,13 +222,27 @@ def plan_for(trajs: np.ndarray, sizes: dict, has_pos: bool = True, edges=BUCKET_
         rate = cache.get(cache_key(device, edge, tpl), {}).get("hops_per_s")
         if rate:
             t_thr = max(t_thr, float(budget[sel].sum()) / rate)
+    # team thresholds from the trajectories' own hop latencies (a probe launch);
+    # fixed thresholds (serving) and batches without positions keep the model
+    probed = None
+    if team_min_hops is None and has_pos:
+        probed = probe_team_thresholds(trajs, bucket, budget, t_thr, edges)
+        if log:
+            log(f"  team thresholds from the latency probe (t_thr {t_thr:.3f} s): {probed}")
     teams, solos = [], []
     for edge, form, w, _ in solo:
         b = edges.index(edge)
         lt = lat.get(edge)
-        entry = team_entry(edge, lt, form, t_thr, budget[bucket == b],
-                           None if team_min_hops is None else team_min_hops.get(edge),
-                           fixed=team_min_hops is not None)
+        if probed is not None:
+            thr = probed.get(edge)
+            entry = None if thr is None else team_entry(edge, lt, form, t_thr, budget[bucket == b],
+                                                        thr, fixed=True, probed=True)
+        else:
+            entry = team_entry(edge, lt, form, t_thr, budget[bucket == b],
+                               None if team_min_hops is None else team_min_hops.get(edge),
+                               fixed=team_min_hops is not None)
+        if log and lt:
+            log(f"  bucket S<={edge}: team entry {entry} (latency tuning {lt})")
         if entry is not None:
             teams.append(entry)
         solos.append((edge, form, w, 1, 0))
@@ -229,6 +251,57 @@ def plan_for(trajs: np.ndarray, sizes: dict, has_pos: bool = True, edges=BUCKET_
               for b, e in enumerate(edges) if (bucket == b).any()}
     return split_team_entries(merge_plan(teams, solos, max_sn, has_pos),
                               int(S_of.max()), int(N_of.max()))
+
+
+def probe_team_thresholds(trajs: np.ndarray, bucket: np.ndarray, budget: np.ndarray,
+                          t_thr: float, edges=BUCKET_EDGES) -> dict:
+    """Per size bucket of two or more site chunks: the team threshold from the
+    trajectories' own hop latencies, or no entry.
+
+    Hop latencies of networks of one size differ several-fold with how often
+    their hops leave the two cached states (1.05 against 0.33 us/hop alone at
+    S = 81 and S = 100 in the 2048-network batch), so one network's latency per
+    bucket misplaces the long trajectories of the others.  A probe launch runs
+    every such trajectory at once (one wave each, positions form) over
+    PROBE_WARM hops and then times PROBE_HOPS more (per-trajectory device
+    clocks of a resumed range: the state has left its start).  With lat_i that
+    time per hop, the batch cannot end before the longest team trajectory,
+    PROBE_TEAM * max(budget_i * lat_i); a trajectory goes to the team entry when
+    its solo run, PROBE_SOLO * budget_i * lat_i, would end after that or after
+    the throughput-bound time t_thr.  The threshold is the smallest budget of
+    such a trajectory (entries route by budget)."""
+    out = {}
+    team_b = [b for b, e in enumerate(edges) if e >= TEAM_MIN_EDGE]
+    idx = np.nonzero(np.isin(bucket, team_b))[0]
+    if idx.size == 0:
+        return out
+    n = idx.size
+    t = trajs[idx].copy()
+    t["prehops"] = 0
+    t["hops"] = PROBE_WARM + PROBE_HOPS
+    b = engine.Batch(t)
+    try:
+        b.set_buckets([(edges[-1], 3, 1, 1, 0)])
+        b.set_state(h_begin=np.zeros(n, np.int64), h_end=np.full(n, PROBE_WARM, np.int64))
+        b.run(precision=L.FP32_BATCHED, resume=True)
+        b.sync()
+        b.set_state(h_begin=np.full(n, PROBE_WARM, np.int64),
+                    h_end=np.full(n, PROBE_WARM + PROBE_HOPS, np.int64))
+        b.run(precision=L.FP32_BATCHED, resume=True)
+        b.sync()
+        r = b.results()
+        lat = b.timing() / PROBE_HOPS
+    finally:
+        b.close()
+    if (r.status != 0).any():
+        raise L.KmcError(L.ERR_ZERO_RATE, "a probed trajectory stopped")
+    work = budget[idx] * lat
+    target = max(t_thr, PROBE_TEAM * float(work.max()))
+    for bb in team_b:
+        m = (bucket[idx] == bb) & (PROBE_SOLO * work > target)
+        if m.any():
+            out[edges[bb]] = int(budget[idx][m].min())
+    return out
 
 
 def team_lds_bytes(form: int, S: int, N: int, team: int) -> int:
@@ -262,12 +335,18 @@ def split_team_entries(plan: list, s_max: int, n_max: int) -> list:
         return plan
     thr = min(teams[0][4], teams[1][4])
     top = max(teams[0][0], teams[1][0])
-    return [(top, form, 1, team, thr, 0, cut + 1), (cut, form, 1, team, thr)] + \
+    # the smaller entry starts above the one-chunk bucket: one-chunk networks
+    # run their long trajectories faster in the solo entry's one-chunk code
+    # (fast hops and leaps) than in a team kernel over two chunks (the
+    # 2048-network batch: 940 against 967-979 ms)
+    low = min(teams[0][0], teams[1][0])
+    s_min = max([e for e in BUCKET_EDGES if e < low], default=0) + 1
+    return [(top, form, 1, team, thr, 0, cut + 1), (cut, form, 1, team, thr, 0, s_min)] + \
         [e for e in plan if e[3] <= 1]
 
 
 def team_entry(edge: int, lt: dict | None, solo_form: int, t_thr: float, budgets: np.ndarray,
-               min_hops: int | None = None, fixed: bool = False):
+               min_hops: int | None = None, fixed: bool = False, probed: bool = False):
     """The team entry of one size bucket, or None.
 
     lt: the bucket's latency tuning ({"form", "team", "lat", "solo": {form: lat},
@@ -282,7 +361,9 @@ def team_entry(edge: int, lt: dict | None, solo_form: int, t_thr: float, budgets
     bucket's throughput form, at BATCH_SLOWDOWN times its lone latency) would
     outlast the batch's throughput-bound time t_thr, and only when some budget
     of the bucket reaches it.  fixed: min_hops is the threshold (serving: a
-    property of the request, not of the mix)."""
+    property of the request, not of the mix).  probed: min_hops came from the
+    trajectories' own latencies (probe_team_thresholds), which decided already
+    that a team pays; the single network's gain test is skipped."""
     if not lt or not lt["solo"] or edge < TEAM_MIN_EDGE:
         return None
     teams = {k: v for k, v in lt.get("teams", {}).items()}
@@ -293,7 +374,7 @@ def team_entry(edge: int, lt: dict | None, solo_form: int, t_thr: float, budgets
     if not teams:
         return None
     tkey, tlat = min(teams.items(), key=lambda kv: kv[1])
-    if tlat * TEAM_MIN_GAIN > min(lt["solo"].values()) * BATCH_SLOWDOWN:
+    if not probed and tlat * TEAM_MIN_GAIN > min(lt["solo"].values()) * BATCH_SLOWDOWN:
         return None
     if fixed:
         thr = min_hops

@@ -13,7 +13,8 @@ namespace kmc {
 // (S*S + 64C float2) and, for BT, the c table (N*S + 64C floats).  PH: the
 // phase-timing instantiation (run_trajectory).
 template <int C, int W, bool REC, bool BT, bool PRIO, bool PH>
-__global__ void __launch_bounds__(64 * W) kmc_batch_fp32_kernel(BatchParams p) {
+__global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(C == 1 ? 4 : 1)))
+kmc_batch_fp32_kernel(BatchParams p) {
   extern __shared__ __align__(16) unsigned char smem[];
   // wave index and everything read from the shared group slot are wave-uniform:
   // readfirstlane lets the compiler keep them (and the trajectory state derived
@@ -59,7 +60,8 @@ __global__ void __launch_bounds__(64 * W) kmc_batch_fp32_kernel(BatchParams p) {
 // Positions policy: every wave pulls its own groups and keeps only its layout's
 // site coordinates in LDS (at most 64*C float4), so a workgroup mixes layouts.
 template <int C, int W, bool REC>
-__global__ void __launch_bounds__(64 * W) kmc_batch_pos_kernel(BatchParams p) {
+__global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(C == 1 ? 4 : 1)))
+kmc_batch_pos_kernel(BatchParams p) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   float4* pos = reinterpret_cast<float4*>(smem) + wave * (64 * C);
@@ -72,10 +74,19 @@ __global__ void __launch_bounds__(64 * W) kmc_batch_pos_kernel(BatchParams p) {
     const LayoutDev L = p.layouts[gd.layout];
     for (int k = lane; k < L.S; k += 64) pos[k] = L.pos4[k];
     wave_lds_sync();
-    PosPolicy<C> pol;
-    pol.init(pos, L, lane);
-    for (int i = 0; i < gd.count; ++i)
-      run_trajectory<C, REC, PosPolicy<C>, 1, true>(p, gd.first + i, L, pol, lane);
+    if (C > 1 && L.S <= 64) {
+      // a one-chunk network: the one-chunk code (fast hops and leaps, fewer
+      // instructions per hop); its trajectories' bits are those of any chunk count
+      PosPolicy<1> pol;
+      pol.init(pos, L, lane);
+      for (int i = 0; i < gd.count; ++i)
+        run_trajectory<1, REC, PosPolicy<1>, 1, true>(p, gd.first + i, L, pol, lane);
+    } else {
+      PosPolicy<C> pol;
+      pol.init(pos, L, lane);
+      for (int i = 0; i < gd.count; ++i)
+        run_trajectory<C, REC, PosPolicy<C>, 1, true>(p, gd.first + i, L, pol, lane);
+    }
     wave_lds_sync();
   }
 }

@@ -210,6 +210,7 @@ struct Engine {
   int device = -1;
   int cu_count = 0;
   int lds_bytes = 0;
+  int wall_khz = 0;   // rate of the kernels' constant-rate clock (wall_clock64)
   char name[256] = {0};
   hipStream_t stream = nullptr;
   std::vector<Layout*> layouts;
@@ -239,6 +240,8 @@ int ensure_init() {
   hipDeviceProp_t prop;
   HIPOK(hipGetDeviceProperties(&prop, dev));
   E.cu_count = prop.multiProcessorCount;
+  HIPOK(hipDeviceGetAttribute(&E.wall_khz, hipDeviceAttributeWallClockRate, dev));
+  if (E.wall_khz <= 0) return fail(KMC_ERR_DEVICE, "no wall-clock rate");
   E.lds_bytes = (int)prop.sharedMemPerBlock;
   snprintf(E.name, sizeof E.name, "%s", prop.gcnArchName);
   HIPOK(hipStreamCreateWithFlags(&E.stream, hipStreamNonBlocking));
@@ -933,6 +936,7 @@ int setup_launch(Batch* B, const kmc_run_opts& o, int form, int W, int team, int
   p.post_hop_record = o.post_hop_record;
   p.resume = o.resume ? 1 : 0;
   p.phases = phases;
+  p.time_limit = o.time_limit_us > 0 ? (int64_t)o.time_limit_us * E.wall_khz / 1000 : 0;
   L.k = k;
   L.grid = grid;
   L.block = 64 * W;
@@ -1167,8 +1171,11 @@ int batch_run_locked(int32_t id, const kmc_run_opts* opts) {
   int st = sync_registry();
   if (st) return st;
   hipStream_t s = E.stream;
+  if (o.time_limit_us < 0) return fail(KMC_ERR_ARGS, "negative time limit %d us", o.time_limit_us);
   if (o.precision == KMC_FP64_DENSE) {
     if (o.resume) return fail(KMC_ERR_UNSUPPORTED, "resume runs use the FP32 batched kernel");
+    if (o.time_limit_us)
+      return fail(KMC_ERR_UNSUPPORTED, "time limits are a feature of the FP32 batched kernel");
     if (o.record & 2)
       return fail(KMC_ERR_UNSUPPORTED, "batched FP64 runs record no traffic (record bit 1); "
                                        "kmc_simulate_discrete records the reference's gross traffic");
@@ -1181,6 +1188,12 @@ int batch_run_locked(int32_t id, const kmc_run_opts* opts) {
     return KMC_OK;
   }
   if (o.precision != KMC_FP32_BATCHED) return fail(KMC_ERR_ARGS, "unknown precision %d", o.precision);
+  if (o.time_limit_us) {
+    // team waves read their own clocks and could disagree on where to stop
+    bool team = B->plan.empty() ? o.team > 1 : false;
+    for (const kmc_bucket& pb : B->plan) team = team || pb.team > 1;
+    if (team) return fail(KMC_ERR_UNSUPPORTED, "time limits apply to solo kernels only (team > 1)");
+  }
   std::vector<Fp32Launch> Ls;
   if ((st = prepare_fp32(B, o, Ls))) return st;
   if (Ls.size() > 1 && (st = ensure_side_streams(B, (int)Ls.size()))) return st;

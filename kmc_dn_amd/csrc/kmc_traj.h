@@ -138,6 +138,9 @@ __device__ __forceinline__ float min_f32(float a, float b) {
 template <int C, bool BT>
 struct TabPolicy {
   static constexpr bool kElec = true;   // electrodes in closed form (elec_sums)
+  // fast hops and leaps over several site chunks (table workgroups are few
+  // per CU: their LDS, not their registers, bounds the waves per SIMD)
+  static constexpr bool kFastChunks = true;
   // coul(N, c) is 0: column N of the c table lies in its zero padding, and column
   // N of the pair table is an electrode column (or, with no electrodes, the
   // padding), c = 0
@@ -190,6 +193,10 @@ struct TabPolicy {
 template <int C>
 struct PosPolicy {
   static constexpr bool kElec = false;   // electrodes are walked as members
+  // no fast hops over several site chunks: positions-form batches are bound
+  // by the waves per SIMD their registers allow (148 against 227 VGPRs at
+  // three chunks); one-chunk trajectories take the one-chunk code
+  static constexpr bool kFastChunks = false;
   static constexpr bool kZeroColumnN = false;
   const float* elw = nullptr;
   float* evc = nullptr;
@@ -394,6 +401,16 @@ __device__ __forceinline__ float pick_lane_value(const float (&x)[C], int site) 
   return r;
 }
 
+template <int C>
+__device__ __forceinline__ double pick_lane_f64(const double (&x)[C], int site) {
+  const int cs = site >> 6, ls = site & 63;
+  double r = readlane_f64(x[0], ls);
+#pragma unroll
+  for (int c = 1; c < C; ++c)
+    if (cs == c) r = readlane_f64(x[c], ls);
+  return r;
+}
+
 // One state of a trajectory's two-entry state cache: what a hop in that
 // state needs of it, all in registers.
 template <int C>
@@ -411,12 +428,13 @@ struct StateEntry {
   int udst;          // rows are sources, the walked set is the targets
   int val;
   // The hop from this state back into the other entry's state, as the leaps
-  // of one-chunk kernels test it (uniform values held in VGPRs):
-  int plast;         // its partner lane, in row psel (-1: unknown since either entry changed)
+  // test it (uniform values held in VGPRs):
+  int plast;         // its partner lane, in row psel; -1: unknown since either entry changed,
+                     // -2: taken once since (the bounds are formed when it is taken again)
   double lo, hi;     // row psel is the selected row iff lo <= target < hi
   double exs;        // exc of row psel
   float plo, phi;    // lane plast is the selected partner iff plo <= rr < phi
-  int dlt;           // per lane: its change of the tallies (+1 destination, -1 source)
+  int dlt[C];        // per lane site: its change of the tallies (+1 destination, -1 source)
 };
 
 // x in VGPRs: an inline-asm result with a VGPR operand counts as varying
@@ -492,6 +510,16 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
     ot[c] = (cont && REC && p.occ_time && s < N) ? p.occ_time[(size_t)t * NMAX + s] : 0.0;
   }
   double time = cont ? o->time : 0.0;
+  // The clock: each hop's FP32 waiting time waits in its hop's lane of the
+  // uniform block (lane h & 63) until a fixed-order FP64 wave sum adds the block
+  // to `time`, at every special hop (uniforms, end of the prehops, recompute)
+  // and at the end of the range, so its bits do not depend on how the block's
+  // hops were taken (general step, fast hop or leap).
+  float dtv = 0.f;
+  auto flush_clock = [&]() __attribute__((always_inline)) {
+    time += readlane_f64(wave_scan_f64((double)dtv), 63);
+    dtv = 0.f;
+  };
   int32_t status = __builtin_amdgcn_readfirstlane(cont ? o->status : (int32_t)ST_OK);
   recompute_energies<C>(se, occ, L, es, lane);
   uint64_t accm[C], elm[C];   // lanes of chunk c holding acceptors / electrodes
@@ -549,7 +577,7 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
     return (int)(n < INT32_MAX ? n : INT32_MAX);
   };
   int next_rc = next_multiple(h_begin);
-  const int h_stop = status == ST_OK ? h_end : h_begin;   // a stopped trajectory stays
+  int h_stop = status == ST_OK ? h_end : h_begin;   // a stopped trajectory stays
   int h = h_begin;
   // uniforms of hops (hh & ~63) + lane: at the start and at every multiple of 64
   auto refill = [&](int hh) {
@@ -583,6 +611,11 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
     }
   };
   int nred = 0;   // team reductions so far
+  // Fast hops and leaps: kernels that record take the general step on every
+  // hop, and so do teams over several site chunks, whose miss-heavy long
+  // trajectories ran their general steps a fifth slower beside the fast path's
+  // registers, and the positions form over several chunks
+  constexpr bool kFast = !REC && (C == 1 || (T == 1 && Pol::kFastChunks));
   StateEntry<C> E0, E1;
   E0.val = E1.val = 0;
   E0.plast = E1.plast = -1;
@@ -677,8 +710,8 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
         W.nzr[c] = in_vgpr((uint64_t)0);
         continue;
       }
-      W.inc[c] = c == 0 ? wave_scan_f64((double)R[c], b15, b31)
-                        : wave_scan_f64((double)R[c], b15, b31) + total;
+      W.inc[c] = c == 0 ? wave_scan_f64((double)R[c])
+                        : wave_scan_f64((double)R[c]) + total;
       W.exc[c] = W.inc[c] - (double)R[c];
       W.nzr[c] = in_vgpr(ballot(R[c] > 0.f));
       total = readlane_f64(W.inc[c], 63);
@@ -703,9 +736,10 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
   // -1 when the trajectory stops.
   auto hop = [&](StateEntry<C>& F, const StateEntry<C>& B) __attribute__((always_inline)) -> int {
     if constexpr (PRIO) __builtin_amdgcn_s_setprio(3);
-    const float lg = readlane_f32(lg_lane, (int)(h & 63));
-    // exponential waiting time -ln(1-u)/total, FP32 quotient into the FP64 clock
-    const double dt = (double)(lg * F.rtot);
+    // exponential waiting time -ln(1-u)/total, an FP32 quotient (the clock
+    // keeps it in the hop's lane; the records take it in FP64)
+    const float dt32 = lg_lane * F.rtot;
+    const double dt = REC ? (double)readlane_f32(dt32, (int)(h & 63)) : 0.0;
     // event uniform x total, multiplied on every lane and read from this hop's
     const double target = readlane_f64(ue_lane * F.tot, (int)(h & 63));
 
@@ -822,7 +856,7 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
       for (int c = 0; c < C; ++c)
         if (__builtin_amdgcn_inverse_ballot_w64(occ[c] & accm[c])) ot[c] += dt;
     }
-    time += dt;
+    if (lane == (h & 63)) dtv = dt32;
     mark(5);
     bool same = B.val != 0;
 #pragma unroll
@@ -830,30 +864,51 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
     return same ? 1 : 0;
   };
 
-  // One hop whose current state's entry is F (B: the entry of the state
-  // before); false: the trajectory stops here or its range ends.
-  auto step = [&](StateEntry<C>& F, StateEntry<C>& B) __attribute__((always_inline)) -> bool {
-    if (__builtin_expect(h == special, 0)) {
-      if ((h & 63) == 0) refill(h);
-      if (h == prehops) {
-        time = 0.0;
-#pragma unroll
-        for (int c = 0; c < C; ++c) {
-          eo[c] = 0;
-          ot[c] = 0.0;
-        }
-      }
-      if (h == next_rc) {
-        recompute_energies<C>(se, occ, L, es, lane);
-        next_rc = next_multiple(h);
-        pend = false;
-        cur_ok = false;
-        E0.val = E1.val = 0;
-      }
-      special = next_special(h, false);
-      h_lim = special < h_stop ? special : h_stop;
+  // The work of a special hop before the hop itself: the clock's block sum,
+  // uniforms, the end of the prehops, the FP64 recompute (which empties the
+  // state cache).
+  // false: the launch's time limit has passed at this recompute point, and the
+  // trajectory stops here (a multiple of the recompute interval, from which a
+  // resumed range continues exactly as one long launch would).  Team waves read
+  // their clocks apart and could disagree, so teams carry no limit (the host
+  // refuses one).
+  auto special_work = [&]() __attribute__((always_inline)) -> bool {
+    if (T == 1 && p.time_limit > 0 && h == next_rc && h > h_begin &&
+        wall_clock64() - t_start > p.time_limit) {
+      h_stop = h;
+      h_lim = h;
+      return false;
     }
+    flush_clock();
+    if ((h & 63) == 0) refill(h);
+    if (h == prehops) {
+      time = 0.0;
+#pragma unroll
+      for (int c = 0; c < C; ++c) {
+        eo[c] = 0;
+        ot[c] = 0.0;
+      }
+    }
+    if (h == next_rc) {
+      recompute_energies<C>(se, occ, L, es, lane);
+      next_rc = next_multiple(h);
+      pend = false;
+      cur_ok = false;
+      E0.val = E1.val = 0;
+    }
+    special = next_special(h, false);
+    h_lim = special < h_stop ? special : h_stop;
     mark(0);
+    return true;
+  };
+
+  // One hop whose current state's entry is F (B: the entry of the state
+  // before), its special work done; false: the trajectory stops here or its
+  // range ends.
+  auto step = [&](StateEntry<C>& F, StateEntry<C>& B) __attribute__((always_inline)) -> bool {
+    if constexpr (!kFast) {
+      if (__builtin_expect(h == special, 0) && !special_work()) return false;
+    }
     if (!cur_ok) {
       // a new state: its energies are the previous state's plus the last hop's
       // rank-1 update (both Coulomb columns loaded together; an electrode end
@@ -905,7 +960,7 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
   // the general step takes the hop.  Kernels that record take the general step
   // on every hop.
   //
-  // Leaps (one-chunk kernels): once the hop back is known from both states
+  // Leaps: once the hop back is known from both states
   // (plast), whether hop k of the current uniform block takes it depends only
   // on its own uniform, so the lanes test the hops of the block in parallel,
   // lane k for hop (h & ~63) + k in the state that hop starts from (F and B
@@ -917,11 +972,10 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
   // those of taking the hops one at a time.
   auto fast_hop = [&](StateEntry<C>& F, const StateEntry<C>& B)
                       __attribute__((always_inline)) -> bool {
-    if constexpr (REC) return false;
     if (h >= h_lim) return false;
     const int hl = h & 63;
     const double tlane = ue_lane * F.tot;   // every lane's hop's target in state F
-    if constexpr (C == 1) {
+    {
       if ((F.plast | B.plast) >= 0) {
         const double tb = ue_lane * B.tot;
         const float rf = (float)(tlane - F.exs), rb = (float)(tb - B.exs);
@@ -937,11 +991,14 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
         if ((unsigned)n > (unsigned)nmax) n = nmax;
         if (n == 0) return false;
         n -= (n & 1) ^ 1;
-        const double dtl = (double)(lg_lane * (__builtin_amdgcn_inverse_ballot_w64(inf)
-                                                   ? F.rtot : B.rtot));
-        for (int k = hl; k < hl + n; ++k) time += readlane_f64(dtl, k);
-        eo[0] += ((n + 1) >> 1) * F.dlt + (n >> 1) * B.dlt;
-        occ[0] = B.key[0];
+        const uint64_t lanes = (~0ull << hl) & (hl + n < 64 ? ~(~0ull << (hl + n)) : ~0ull);
+        const float dtl = lg_lane * (__builtin_amdgcn_inverse_ballot_w64(inf) ? F.rtot : B.rtot);
+        if (__builtin_amdgcn_inverse_ballot_w64(lanes)) dtv = dtl;
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+          eo[c] += ((n + 1) >> 1) * F.dlt[c] + (n >> 1) * B.dlt[c];
+          occ[c] = B.key[c];
+        }
         h += n;
         if constexpr (PH) {
           ph[7] += n;
@@ -1010,19 +1067,36 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
       const int s = lane + 64 * c;
       eo[c] += (s == dst ? 1 : 0) - (s == src ? 1 : 0);
     }
-    time += (double)(readlane_f32(lg_lane, hl) * F.rtot);
+    if (lane == hl) dtv = lg_lane * F.rtot;
     ++h;
-    if constexpr (C == 1) {
-      if (F.plast < 0) {
-        // the hop back from F is known: its row's and partner's bounds
-        F.plast = pick;
-        F.hi = in_vgpr(readlane_f64(F.inc[0], sel));
-        F.exs = in_vgpr(readlane_f64(F.exc[0], sel));
-        F.lo = in_vgpr(wave_max_f64(lane < sel ? F.inc[0] : -INFINITY));
-        F.phi = in_vgpr(readlane_f32(F.pcum[0], pick));
-        F.plo = in_vgpr(wave_max_f32(lane < pick && ((F.pnz[0] >> lane) & 1ull)
-                                         ? F.pcum[0] : -INFINITY));
-        F.dlt = (lane == dst ? 1 : 0) - (lane == src ? 1 : 0);
+    if (F.plast == -1) {
+      // the first hop back from F since the entries changed: at a high miss
+      // rate most pairs of states never come to a leap, so the bounds wait
+      // for a second one
+      F.plast = -2;
+    } else if (F.plast < 0) {
+      // the hop back from F is known: its row's and partner's bounds (maxima
+      // over the sites before the row and before the partner)
+      F.plast = pick;
+      double lo = -INFINITY;
+      float plo = -INFINITY;
+#pragma unroll
+      for (int c = 0; c < C; ++c) {
+        if (c > 0 && 64 * c >= S) break;
+        const int s = lane + 64 * c;
+        lo = fmax(lo, wave_max_f64(s < sel ? F.inc[c] : -INFINITY));
+        plo = fmaxf(plo, wave_max_f32(s < pick && ((F.pnz[c] >> lane) & 1ull)
+                                          ? F.pcum[c] : -INFINITY));
+      }
+      F.lo = in_vgpr(lo);
+      F.plo = in_vgpr(plo);
+      F.hi = in_vgpr(pick_lane_f64<C>(F.inc, sel));
+      F.exs = in_vgpr(pick_lane_f64<C>(F.exc, sel));
+      F.phi = in_vgpr(pick_lane_value<C>(F.pcum, pick));
+#pragma unroll
+      for (int c = 0; c < C; ++c) {
+        const int s = lane + 64 * c;
+        F.dlt[c] = (s == dst ? 1 : 0) - (s == src ? 1 : 0);
       }
     }
     if constexpr (PH) {
@@ -1033,34 +1107,47 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
     return true;
   };
 
-  // Hops alternate between the entries.  `one`: entry E1 holds (or, after a
-  // miss, is to hold) the current state.  A general step, then fast hops for
-  // as long as they apply (the current state's entry must hold it).
-  bool one = false;
-  while (h < h_stop) {
-    if (!one) {
-      if (!step(E0, E1)) break;
-    } else {
-      if (!step(E1, E0)) break;
-    }
-    one = !one;
-    if (!cur_ok) continue;
-    if (one) {
-      while (fast_hop(E1, E0))
-        if (!fast_hop(E0, E1)) {
-          one = false;
-          break;
+  if constexpr (kFast) {
+    // Hops alternate between the entries.  `one`: entry E1 holds (or, after a
+    // miss, is to hold) the current state.  A special hop's work first, then fast
+    // hops and leaps for as long as they apply (the current state's entry must
+    // hold it), then a general step for the hop that stopped them.
+    bool one = false;
+    while (h < h_stop) {
+      if (h == special && !special_work()) break;
+      if (cur_ok) {
+        if (one) {
+          while (fast_hop(E1, E0))
+            if (!fast_hop(E0, E1)) {
+              one = false;
+              break;
+            }
+        } else {
+          while (fast_hop(E0, E1))
+            if (!fast_hop(E1, E0)) {
+              one = true;
+              break;
+            }
         }
-    } else {
-      while (fast_hop(E0, E1))
-        if (!fast_hop(E1, E0)) {
-          one = true;
-          break;
-        }
+        if (h >= h_lim) continue;   // the range's end or a special hop
+      }
+      if (!one) {
+        if (!step(E0, E1)) break;
+      } else {
+        if (!step(E1, E0)) break;
+      }
+      one = !one;
     }
+  } else {
+    // general steps only, unrolled by the entries' roles
+    if (h < h_stop)
+      for (;;) {
+        if (!step(E0, E1)) break;
+        if (!step(E1, E0)) break;
+      }
   }
-
   if constexpr (PRIO) __builtin_amdgcn_s_setprio(0);   // a stopped trajectory leaves the loop raised
+  flush_clock();
   if (!team.writer()) return;
   if constexpr (PH) {
     if (p.phases && lane == 0)

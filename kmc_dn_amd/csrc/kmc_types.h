@@ -124,6 +124,8 @@ struct BatchParams {
   int32_t resume;               // 1: run [h_begin, h_end) from the state in `out`
   uint64_t* phases;             // [FP32_PHASES] sums over trajectories (the phase-timing
                                 // instantiations only), or null
+  int64_t time_limit;           // constant-rate clock ticks a solo trajectory runs before
+                                // it stops at its next recompute point; 0: none
 };
 
 // Hop phases of the FP32 batched kernels timed by their phase-timing

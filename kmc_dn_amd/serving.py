@@ -159,6 +159,11 @@ class Server:
         self.latency = {}
         self.plan = [tuple(int(x) for x in p) for p in plan] if plan is not None \
             else self._tune(tune_log)
+        # rounds end at a device time limit per trajectory; team kernels carry none
+        # (their waves would have to agree on where to stop), so plans with team
+        # entries size each round by its hop ranges alone
+        teams = any(len(e) > 3 and e[3] > 1 for e in self.plan)
+        self.time_limit_us = 0 if teams else int(self.round_s * 1e6)
 
         self.batch = None
         self.pending = collections.deque()
@@ -341,13 +346,22 @@ class Server:
         return C * (16 + walk) / gain
 
     def quotas(self, idx: np.ndarray) -> np.ndarray:
-        """Hop range per trajectory this round: the round's target time over the
-        trajectory's seconds per hop (measured in its previous round, else the cost
-        model times the current calibration), in multiples of the recompute
-        interval, so that every trajectory of a round ends at about the same time."""
+        """Hop range per trajectory this round: twice the round's target time over
+        the trajectory's seconds per hop (measured in its previous round, else the
+        cost model times the current calibration), in multiples of the recompute
+        interval.  The device time limit, not the range, ends a round: a hop's
+        latency varies several-fold along a trajectory (runs of hops between two
+        cached states against hops into new states), and one trajectory whose
+        range was sized on a fast round held a whole round up to ten times its
+        target; the range only keeps a trajectory from running past a round far
+        beyond its measured pace.  Without a time limit (plans with team entries)
+        the range is the round's target time over that pace."""
         lat = self.s_lat[idx]
-        q = (self.round_s / lat).astype(np.int64) // self.K * self.K
-        return np.clip(q, self.K, 16 * self.chunk)
+        if not getattr(self, "time_limit_us", 0):   # the ranges alone size the round
+            q = (self.round_s / lat).astype(np.int64) // self.K * self.K
+            return np.clip(q, self.K, 16 * self.chunk)
+        q = (2.0 * self.round_s / lat).astype(np.int64) // self.K * self.K
+        return np.clip(q, self.K, 64 * self.chunk)
 
     def plan_entry(self, S: np.ndarray, budget: np.ndarray) -> np.ndarray:
         """Index of the plan entry each trajectory runs in: the first entry with
@@ -391,12 +405,18 @@ class Server:
         b = self.batch
         b.update(trajs)
         b.set_state(hb, he, self.s_time[idx], self.s_eo[idx], self.s_occ[idx])
-        b.run(precision=L.FP32_BATCHED, resume=True, recompute_interval=self.K)
+        # the device time limit ends the round: a trajectory stops at its first
+        # recompute point past it and continues there next round
+        b.run(precision=L.FP32_BATCHED, resume=True, recompute_interval=self.K,
+              time_limit_us=self.time_limit_us)
         t1 = time.perf_counter()
         self.round_ms.append(b.sync())
         t2 = time.perf_counter()
         r = b.results()
-        ran = he - hb
+        # hops reached: the range's end, or the recompute point where the time
+        # limit stopped the trajectory (the budget for one that stopped early)
+        reached = np.where(r.status != 0, self.s_hops[idx], r.hops_done)
+        ran = reached - hb
         sec = b.timing()
         m = (ran >= 256) & (sec > 0)
         if m.any():
@@ -409,8 +429,8 @@ class Server:
         self.s_eo[idx] = r.eo
         self.s_occ[idx] = r.occ_bits
         self.s_status[idx] = r.status
-        self.s_h[idx] = np.where(r.status != 0, self.s_hops[idx], he)
-        self.hops_done += int(np.sum(he - hb))
+        self.s_h[idx] = reached
+        self.hops_done += int(np.sum(ran))
         self.rounds += 1
         self._retire(clock)
         t3 = time.perf_counter()

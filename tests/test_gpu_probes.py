@@ -68,8 +68,13 @@ def test_fp32_phase_instantiation_matches(gpu_engine, N, P, form, team):
         _same(r0, b.results())
         total = K * hops
         assert 0 < cyc[7] <= total and 0 < cyc[8] <= total and cyc[9] <= cyc[8]
-        for k in (0, 3, 4, 5, 6):
+        for k in (0, 3, 4, 5):
             assert cyc[k] > 0, probe.FP32_PHASES[k]
+        # fast hops and leaps: one-chunk kernels and table-form solo kernels
+        if N + P <= 64 or (team == 1 and form != 3):
+            assert cyc[6] > 0, probe.FP32_PHASES[6]
+        else:
+            assert cyc[6] == 0 and cyc[9] == 0
         if cyc[7] < total:
             assert cyc[1] > 0 and cyc[2] > 0
     finally:
