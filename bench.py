@@ -37,16 +37,19 @@ METRIC = "hops/sec, 4096-traj N=30 ensemble vs plain; p50/p99 latency of served 
 # divides by the LDS read peak measured in the same run (ds_read_b128 stream probe).
 LDS_PEAK_SPEC_GBS = 256 * 256 * 2.4
 # HBM bytes per launch of the batched kernel on the default workload (4096 x 1e5 hops,
-# N=30/P=8, auto table form and workgroup size): rocprofv3 --pmc FETCH_SIZE 384.812 KiB
-# and WRITE_SIZE 640 KiB per dispatch, scaled by the copy-kernel calibration factors
-# 2.0 (FETCH) and 1.0 (WRITE): (2 * 384.812 + 640) * 1024 bytes.
-HBM_TRAFFIC_BYTES = 1_443_455
-# VALU instructions per hop and wave of the batched kernel on the default workload
-# (rocprofv3 --pmc SQ_INSTS_VALU: 4.288e10 per dispatch of 4.096e8 hops), and the VALU
-# issue peak: one wave64 instruction per 4 cycles on each of a CU's 4 SIMDs (16 lanes),
-# i.e. one per CU and cycle, 256 CUs at 2.4 GHz.
-VALU_INSTS_PER_HOP = 104.7
-VALU_ISSUE_PEAK = 256 * 2.4e9
+# N=30/P=8, auto table form and workgroup size): rocprofv3 --pmc FETCH_SIZE 5245.81 KiB
+# and WRITE_SIZE 13952 KiB per dispatch, scaled by the copy-kernel calibration factors
+# 2.0 (FETCH) and 1.0 (WRITE): (2 * 5245.81 + 13952) * 1024 bytes -- the launch's own
+# I/O (1.1 MB) and register-spill scratch (the one-chunk kernels are held to 128 VGPRs).
+HBM_TRAFFIC_BYTES = 25_030_267
+# VALU and scalar instructions per hop and wave of the batched kernel on the default
+# workload (rocprofv3 --pmc SQ_INSTS_VALU 3.099e9, SQ_INSTS_SALU 2.427e9 per dispatch of
+# 4.096e8 hops), and the issue peak of each: one wave64 VALU instruction per 4 cycles on
+# each of a CU's 4 SIMDs, and one scalar instruction per cycle from the CU's shared
+# scalar unit, i.e. one per CU and cycle each, 256 CUs at 2.4 GHz.
+VALU_INSTS_PER_HOP = 7.57
+SALU_INSTS_PER_HOP = 5.92
+ISSUE_PEAK = 256 * 2.4e9
 
 
 def lds_bytes_per_hop(N: int, P: int) -> int:
@@ -424,13 +427,19 @@ def main() -> int:
             "bytes_per_hop": lds_bytes_per_hop(net.N, net.P),
             "hops_per_launch": hops_per_step,
             "kernel_ms": float(np.mean(kernel_ms)),
-            "valu_issue_frac": (VALU_INSTS_PER_HOP * hops_per_step / avg_kernel_s / VALU_ISSUE_PEAK
+            "valu_issue_frac": (VALU_INSTS_PER_HOP * hops_per_step / avg_kernel_s / ISSUE_PEAK
                                 if default_workload else None),
-            "note": "algorithmic LDS bytes of the dense formulation / kernel time; peak = "
-                    "ds_read_b128 stream measured in this run; the kernel is bound by the VALU "
-                    "its 4 waves per SIMD share (VALUBusy 82%, LdsUtil 10%; 104.7 VALU and 81.7 scalar "
-                    "instructions per hop): valu_issue_frac = profiled VALU "
-                    "instructions per hop x hops / kernel time / (1 per CU and cycle)",
+            "salu_issue_frac": (SALU_INSTS_PER_HOP * hops_per_step / avg_kernel_s / ISSUE_PEAK
+                                if default_workload else None),
+            "note": "achieved = algorithmic LDS bytes of the reference's dense formulation "
+                    "(9,768 B per hop at N=30, P=8) x hops / kernel time; peak = ds_read_b128 "
+                    "stream measured in this run.  frac exceeds 1 because 97% of the hops never "
+                    "read those tables: a two-entry state cache serves 98.9% of the hops and "
+                    "leaps take the runs of hops between its two states a whole uniform block "
+                    "at a time.  The kernel is bound by the latency of each wave's hop chain "
+                    "(LdsUtil 1%, VALUBusy 35%, waves waiting on dependencies 42% of their "
+                    "cycles); valu/salu_issue_frac = profiled instructions per hop (7.57 VALU, "
+                    "5.92 scalar) x hops / kernel time / (one per CU and cycle)",
         },
         "plain_fp64": {"value": plain_value, "unit": "hops/s",
                        "kernel_ms": float(np.mean(pms)),

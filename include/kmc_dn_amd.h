@@ -44,7 +44,9 @@ enum {
   KMC_ERR_NOMEM = 3,       /* device or host allocation failed */
   KMC_ERR_DEVICE = 4,      /* HIP runtime error */
   KMC_ERR_UNSUPPORTED = 5, /* entry point / mode outside this engine's scope */
-  KMC_ERR_LIMIT = 6        /* network larger than the kernel's limits */
+  KMC_ERR_LIMIT = 6,       /* network larger than the kernel's limits */
+  KMC_ERR_TIMEOUT = 7,     /* kmc_batch_wait: the launch has not ended within the timeout */
+  KMC_ERR_CANCELLED = 8    /* serving: a request cancelled or past its deadline (host loop) */
 };
 
 /* Precision / kernel of a batch run. */
@@ -252,6 +254,12 @@ int kmc_batch_set_buckets(int32_t batch, const kmc_bucket* buckets, int32_t nbuc
 int kmc_batch_run(int32_t batch, const kmc_run_opts* opts);
 /* Waits for the last run; elapsed_ms (optional) = HIP-event time of its kernels. */
 int kmc_batch_sync(int32_t batch, double* elapsed_ms);
+/* Host watchdog: waits at most timeout_ms for the last run to end, polling its
+ * completion event without blocking the device.  KMC_OK (and elapsed_ms as in
+ * kmc_batch_sync) when it ended, KMC_ERR_TIMEOUT when it had not; the launch is
+ * left running either way (a later wait or sync still collects it; a launch
+ * with kmc_run_opts.time_limit_us ends by itself).  No reference counterpart. */
+int kmc_batch_wait(int32_t batch, double timeout_ms, double* elapsed_ms);
 /* Copies results to host arrays (each may be NULL):
  * time[ntraj], eo[ntraj*KMC_PMAX], occ[ntraj*KMC_OCCW], status[ntraj],
  * hops_done[ntraj], occ_time[ntraj*KMC_NMAX] (only after a record run). */

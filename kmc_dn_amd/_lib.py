@@ -18,7 +18,8 @@ NMAX = 256
 PMAX = 16
 OCCW = 4
 
-OK, ERR_ARGS, ERR_ZERO_RATE, ERR_NOMEM, ERR_DEVICE, ERR_UNSUPPORTED, ERR_LIMIT = range(7)
+(OK, ERR_ARGS, ERR_ZERO_RATE, ERR_NOMEM, ERR_DEVICE, ERR_UNSUPPORTED, ERR_LIMIT, ERR_TIMEOUT,
+ ERR_CANCELLED) = range(9)
 FP64_DENSE, FP32_BATCHED = 0, 1
 
 
@@ -90,6 +91,7 @@ SIGNATURES = {
     "kmc_batch_create": (C.c_int, [C.POINTER(kmc_traj), _i64, _i32p]),
     "kmc_batch_run": (C.c_int, [_i32, C.POINTER(kmc_run_opts)]),
     "kmc_batch_sync": (C.c_int, [_i32, _dp]),
+    "kmc_batch_wait": (C.c_int, [_i32, C.c_double, _dp]),
     "kmc_batch_results": (C.c_int, [_i32, _dp, _i64p, _u64p, _i32p, _i64p, _dp]),
     "kmc_batch_traffic": (C.c_int, [_i32, _dp, _i64p]),
     "kmc_batch_timing": (C.c_int, [_i32, _dp]),

@@ -11,7 +11,9 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <chrono>
 #include <mutex>
+#include <thread>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -1415,6 +1417,34 @@ int kmc_batch_sync(int32_t batch, double* elapsed_ms) {
     if (!B) return fail(KMC_ERR_ARGS, "bad batch id %d", batch);
   }
   HIPOK(hipEventSynchronize(B->ev1));
+  if (elapsed_ms) {
+    float ms = 0.f;
+    HIPOK(hipEventElapsedTime(&ms, B->ev0, B->ev1));
+    *elapsed_ms = ms;
+  }
+  return KMC_OK;
+}
+
+int kmc_batch_wait(int32_t batch, double timeout_ms, double* elapsed_ms) {
+  Engine& E = eng();
+  Batch* B;
+  {
+    std::lock_guard<std::mutex> lk(E.mu);
+    B = get_batch(batch);
+    if (!B) return fail(KMC_ERR_ARGS, "bad batch id %d", batch);
+  }
+  if (!(timeout_ms >= 0.0)) return fail(KMC_ERR_ARGS, "timeout must be >= 0 ms");
+  const auto t0 = std::chrono::steady_clock::now();
+  for (;;) {
+    const hipError_t q = hipEventQuery(B->ev1);
+    if (q == hipSuccess) break;
+    if (q != hipErrorNotReady) return fail(KMC_ERR_DEVICE, "event query: %s", hipGetErrorString(q));
+    const double ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (ms >= timeout_ms)
+      return fail(KMC_ERR_TIMEOUT, "batch %d still running after %.3f ms", batch, ms);
+    std::this_thread::sleep_for(std::chrono::microseconds(ms < 2.0 ? 20 : 200));
+  }
   if (elapsed_ms) {
     float ms = 0.f;
     HIPOK(hipEventElapsedTime(&ms, B->ev0, B->ev1));

@@ -209,6 +209,16 @@ class Batch:
         L.check(L.load().kmc_batch_sync(self.id, C.byref(ms)))
         return ms.value
 
+    def wait(self, timeout_ms: float) -> float | None:
+        """Watchdog wait: the kernel time (ms) when the last run ended within
+        timeout_ms, None when it had not (it keeps running)."""
+        ms = C.c_double()
+        rc = L.load().kmc_batch_wait(self.id, float(timeout_ms), C.byref(ms))
+        if rc == L.ERR_TIMEOUT:
+            return None
+        L.check(rc)
+        return ms.value
+
     def results(self) -> BatchResult:
         n = self.n
         t = np.zeros(n)

@@ -24,6 +24,12 @@ voltage_search.py:102-157).
   server's lifetime).
 * A request whose trajectory stops (zero total rate) or whose inputs are
   rejected completes with its status; the others are not affected.
+* A request cancelled by the caller (Server.cancel) or past its deadline
+  (ServeRequest.deadline_s after its arrival) completes at the next round
+  boundary with status ERR_CANCELLED and its tallies so far; its slots and
+  energy sets return to the pools.  A watchdog bounds every round's wait: a
+  round whose launch has not ended within watchdog_s raises (the device
+  stopped making progress; rounds end by themselves at their time limit).
 """
 from __future__ import annotations
 
@@ -64,6 +70,7 @@ class ServeRequest:
     V: np.ndarray                 # (P,) or (k, P) voltage vectors
     hops: int
     replicas: int = 1
+    deadline_s: float | None = None   # seconds after t_arrive; None: none
 
 
 @dataclass
@@ -114,7 +121,8 @@ class Server:
 
     def __init__(self, nets, bases, chunk_hops: int = 8192, max_active: int = 8192,
                  mem_budget: int = 64 << 20, plan=None, recompute_interval: int = 1024,
-                 eset_block: int = 64, team_min_hops: int | None = None, tune_log=None):
+                 eset_block: int = 64, team_min_hops: int | None = None, tune_log=None,
+                 watchdog_s: float = 30.0):
         if chunk_hops <= 0 or chunk_hops % recompute_interval:
             raise ValueError("chunk_hops must be a positive multiple of recompute_interval")
         self.nets, self.bases = list(nets), list(bases)
@@ -175,6 +183,8 @@ class Server:
         self.round_stats = []     # (trajectories, kernel ms, slowest and median trajectory ms)
         self.hops_done = 0
         self._next = 0
+        self.watchdog_s = watchdog_s
+        self.cancelled = set()       # request ids to cancel at the next round boundary
 
     # ---- set-up -------------------------------------------------------------
     def _tune(self, log):
@@ -228,6 +238,37 @@ class Server:
 
     def busy(self) -> bool:
         return bool(self.pending or self.active)
+
+    def cancel(self, rid: int) -> None:
+        """Cancel request `rid` (pending or running) at the next round boundary."""
+        self.cancelled.add(int(rid))
+
+    def _late(self, req, now: float) -> bool:
+        return int(req.rid) in self.cancelled or (
+            req.deadline_s is not None and now - req.t_arrive > req.deadline_s)
+
+    def _expire(self, now: float) -> None:
+        """Complete cancelled requests and those past their deadline: pending ones
+        without running, running ones with their tallies so far."""
+        if not self.cancelled and not any(r.deadline_s is not None for r in self.pending) and \
+                not any(a.req.deadline_s is not None for a in self.active.values()):
+            return
+        keep = collections.deque()
+        for req in self.pending:
+            if self._late(req, now):
+                self._fail(req, np.atleast_2d(np.asarray(req.V, dtype=np.float64)), now,
+                           L.ERR_CANCELLED, f"request {req.rid} cancelled before it ran")
+                self.cancelled.discard(int(req.rid))
+            else:
+                keep.append(req)
+        self.pending = keep
+        for rq in [k for k, a in self.active.items() if self._late(a.req, now)]:
+            act = self.active[rq]
+            sl = act.slots
+            ran, total = int(self.s_h[sl].sum()), int(self.s_hops[sl].sum())
+            self._complete(rq, now, L.ERR_CANCELLED,
+                           f"request {act.req.rid} cancelled after {ran} of {total} hops")
+            self.cancelled.discard(int(act.req.rid))
 
     def _admit(self, now: float) -> None:
         """Admit the longest FIFO prefix of pending requests that fits, in bulk: one
@@ -386,6 +427,7 @@ class Server:
 
     def round(self, clock) -> int:
         """Admit, run one round, retire finished requests; returns trajectories run."""
+        self._expire(clock())
         self._admit(clock())
         idx = np.nonzero(self.s_live & (self.s_h < self.s_hops))[0]
         self._retire(clock)
@@ -410,7 +452,11 @@ class Server:
         b.run(precision=L.FP32_BATCHED, resume=True, recompute_interval=self.K,
               time_limit_us=self.time_limit_us)
         t1 = time.perf_counter()
-        self.round_ms.append(b.sync())
+        ms = b.wait(self.watchdog_s * 1e3)
+        if ms is None:
+            raise RuntimeError(f"serving round {self.rounds} did not end within "
+                               f"{self.watchdog_s} s ({n} trajectories)")
+        self.round_ms.append(ms)
         t2 = time.perf_counter()
         r = b.results()
         # hops reached: the range's end, or the recompute point where the time
@@ -450,23 +496,30 @@ class Server:
             sl = act.slots
             if not (self.s_h[sl] >= self.s_hops[sl]).all():
                 continue
-            req = act.req
-            k, rep = req.V.shape[0], max(1, int(req.replicas))
-            P = self.nets[req.layout].P
-            eo = self.s_eo[sl].reshape(k, rep, L.PMAX).copy()
-            tm = self.s_time[sl].reshape(k, rep).copy()
             st = self.s_status[sl]
-            status = int(st[st != 0][0]) if (st != 0).any() else 0
-            with np.errstate(divide="ignore", invalid="ignore"):
-                cur = eo[:, :, :P] / tm[:, :, None]
-            self.completed.append(Completed(req.rid, req.t_arrive, act.t_admit, now, status, cur,
-                                            eo, tm, self.s_occ[sl].reshape(k, rep, L.OCCW).copy()))
-            self.s_live[sl] = False
-            self.free_slots.extend(int(x) for x in sl)
-            for es in act.esets:
-                self._give_eset(req.layout, es)
-            self.mem_used -= act.bytes
-            del self.active[int(rq)]
+            self._complete(int(rq), now, int(st[st != 0][0]) if (st != 0).any() else 0)
+
+    def _complete(self, rq: int, now: float, status: int, message: str = "") -> None:
+        """Completes active request rq with its trajectories' current tallies and
+        returns its slots and energy sets."""
+        act = self.active[rq]
+        sl = act.slots
+        req = act.req
+        k, rep = req.V.shape[0], max(1, int(req.replicas))
+        P = self.nets[req.layout].P
+        eo = self.s_eo[sl].reshape(k, rep, L.PMAX).copy()
+        tm = self.s_time[sl].reshape(k, rep).copy()
+        with np.errstate(divide="ignore", invalid="ignore"):
+            cur = eo[:, :, :P] / tm[:, :, None]
+        self.completed.append(Completed(req.rid, req.t_arrive, act.t_admit, now, status, cur,
+                                        eo, tm, self.s_occ[sl].reshape(k, rep, L.OCCW).copy(),
+                                        message))
+        self.s_live[sl] = False
+        self.free_slots.extend(int(x) for x in sl)
+        for es in act.esets:
+            self._give_eset(req.layout, es)
+        self.mem_used -= act.bytes
+        del self.active[rq]
 
     def close(self) -> None:
         if self.batch is not None:

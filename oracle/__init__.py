@@ -28,7 +28,9 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
+# KMC_ORACLE_LIB: another build of the same source (the sanitizer build of
+# tests/test_sanitizers.py)
+LIB_PATH = os.environ.get("KMC_ORACLE_LIB") or os.path.join(_HERE, "_build", "liboracle.so")
 _lib = None
 
 _dp = C.POINTER(C.c_double)

@@ -119,3 +119,48 @@ def test_poison_requests_fail_alone(resident):
             _assert_same(d[c.rid], c)
     finally:
         srv.close()
+
+
+def test_deadlines_and_cancellation_fail_alone(resident):
+    """A request past its deadline, one cancelled while it runs and one cancelled
+    before it runs complete with ERR_CANCELLED (the running ones with their
+    tallies so far); the others get the results they get alone.  A solo plan, so
+    the rounds end at their device time limit."""
+    nets, bases = resident
+    solo = [(64, 1, 4, 1, 0), (128, 3, 4, 1, 0)]
+    srv = serving.Server(nets, bases, chunk_hops=2048, plan=solo)
+    try:
+        assert srv.time_limit_us > 0
+        good = _stream(nets, 10, 1e4, seed=7)
+        V = np.zeros(8)
+        late = serving.ServeRequest(200, 0.0, 0, V, 3_000_000, deadline_s=0.05)
+        victim = serving.ServeRequest(201, 0.0, 1, V, 3_000_000)
+        never = serving.ServeRequest(202, 0.0, 2, V, 5000)
+        for r in good:
+            r.t_arrive = 0.0
+        t0 = __import__("time").perf_counter()
+
+        def clock():
+            return __import__("time").perf_counter() - t0
+
+        for r in good + [late, victim]:
+            srv.submit(r)
+        print("[gpu] serving: deadline and cancellation", flush=True)
+        srv.round(clock)
+        srv.cancel(201)
+        srv.cancel(202)
+        srv.submit(never)
+        while srv.busy():
+            srv.round(clock)
+            assert clock() < 60
+        d = _by_rid(srv.completed)
+        for rid in (200, 201, 202):
+            assert d[rid].status == _lib.ERR_CANCELLED, (rid, d[rid].message)
+        assert (d[200].time > 0).all() and (d[201].time > 0).all()
+        assert (d[202].time == 0).all()
+        assert srv.mem_used == 0 and not srv.active
+        alone, _ = serving.serve_one_launch_per_request(srv, good, realtime=False)
+        for c in alone:
+            _assert_same(d[c.rid], c)
+    finally:
+        srv.close()

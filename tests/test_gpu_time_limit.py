@@ -85,3 +85,25 @@ def test_time_limits_are_refused_where_unsupported(gpu_engine):
         b.close()
     finally:
         lay.close()
+
+
+def test_watchdog_wait(gpu_engine):
+    """kmc_batch_wait returns KMC_ERR_TIMEOUT (None here) while a launch runs and
+    leaves it running; a later wait collects it with the kernel time of sync."""
+    lay, trajs = _ensemble(4096, 300_000)
+    try:
+        b = engine.Batch(trajs)
+        b.run(precision=_lib.FP32_BATCHED)
+        first = b.wait(0.0)
+        ms = b.wait(60_000.0)
+        print(f"[gpu] watchdog: first wait {first}, kernel {ms:.1f} ms", flush=True)
+        assert ms is not None and ms > 0
+        if ms > 20.0:        # a launch that long cannot have ended at once
+            assert first is None
+        assert b.sync() == ms
+        assert (b.results().status == 0).all()
+        with pytest.raises(_lib.KmcError):
+            _lib.check(_lib.load().kmc_batch_wait(b.id, -1.0, None))
+        b.close()
+    finally:
+        lay.close()
