@@ -328,8 +328,9 @@ double wrapperSimulateCombined(int64_t, int64_t, double, double, double, double,
 double wrapperSimulateProbability(int64_t, int64_t, double, double, double, double, kmc_GoSlice,
                                   kmc_GoSlice, kmc_GoSlice, kmc_GoSlice, kmc_GoSlice, kmc_GoSlice,
                                   int64_t, uint8_t, kmc_GoSlice, kmc_GoSlice);
-/* simulationWrapper.go:171-216, a statistic of the Go occupation cache, which
- * this engine does not keep: returns NaN. */
+/* simulationWrapper.go:171-216, a statistic of the Go occupation map (every
+ * state a trajectory visits), which this engine does not keep (its kernels hold
+ * the last two states only): returns NaN. */
 double analyzeStateOverlap(int64_t, int64_t, double, double, double, double, kmc_GoSlice,
                            kmc_GoSlice, kmc_GoSlice, kmc_GoSlice, kmc_GoSlice, kmc_GoSlice, int64_t,
                            uint8_t, kmc_GoSlice, kmc_GoSlice);

@@ -236,8 +236,9 @@ double wrapperSimulateProbability(int64_t NSites, int64_t NElectrodes, double nu
   return result;
 }
 
-// simulationWrapper.go:171-216 prints state-overlap statistics of the Go cache
-// and returns 0: a diagnostic of that cache, which this engine does not have.
+// simulationWrapper.go:171-216 prints state-overlap statistics of the Go
+// occupation map and returns 0: a diagnostic of a map of every visited state,
+// which this engine does not keep (its kernels hold the last two states).
 double analyzeStateOverlap(int64_t, int64_t, double, double, double, double, GoSlice, GoSlice,
                            GoSlice, GoSlice, GoSlice, GoSlice, int64_t, uint8_t, GoSlice, GoSlice) {
   return NAN;
