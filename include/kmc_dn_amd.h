@@ -53,8 +53,9 @@ enum {
 enum {
   KMC_FP64_DENSE = 0,  /* reference arithmetic: dense S^2 FP64 rates, full energy
                           recompute, sequential row-major prefix sum */
-  KMC_FP32_BATCHED = 1 /* FP32 log-domain rates, O(N) rank-1 energy update with a
-                          periodic FP64 recompute, FP64 wave scan + binary pick */
+  KMC_FP32_BATCHED = 1 /* FP32 log-domain rates, energies from a periodic FP64
+                          recompute plus the Coulomb columns of the sites changed
+                          since, FP64 wave scan + binary pick */
 };
 
 const char* kmc_last_error(void);
@@ -215,6 +216,15 @@ typedef struct kmc_run_opts {
                                  the duration of a launch (schedulers, watchdogs).
                                  0 = none.  Refused with team kernels.  No reference
                                  counterpart. */
+  int32_t state_store;        /* FP32 one-chunk solo kernels (S <= 64): LDS slots
+                                 per trajectory for states evicted from its two-state
+                                 cache, restored instead of re-evaluated when the
+                                 trajectory returns to them.  0 = as many (at most 4)
+                                 as fit without lowering the resident workgroups per
+                                 CU, -1 = none, 1..4 = at most that many.  Changes no
+                                 result, only the time per hop.  No reference
+                                 counterpart (the Go library's occupation cache,
+                                 simulation.go:254-296, keeps every state). */
 } kmc_run_opts;
 
 /* Uploads trajectory descriptors and allocates device outputs. */

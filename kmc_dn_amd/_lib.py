@@ -40,7 +40,8 @@ class kmc_run_opts(C.Structure):
                 ("recompute_interval", C.c_int32), ("record", C.c_int32),
                 ("post_hop_record", C.c_int32), ("grid_limit", C.c_int32),
                 ("use_graph", C.c_int32), ("table_form", C.c_int32),
-                ("resume", C.c_int32), ("team", C.c_int32), ("time_limit_us", C.c_int32)]
+                ("resume", C.c_int32), ("team", C.c_int32), ("time_limit_us", C.c_int32),
+                ("state_store", C.c_int32)]
 
 
 class kmc_bucket(C.Structure):

@@ -8,7 +8,8 @@
 namespace kmc {
 
 // Table policies: the workgroup's W waves run W trajectories of one layout and
-// share its tables in LDS.  Layout: the waves' electrode factors (W*2*PMAX
+// share its tables in LDS.  Layout: the waves' state stores (W *
+// store_wave_bytes(p.store_slots)), the waves' electrode factors (W*2*PMAX
 // floats), the electrode weights (elw_lds_floats), the pair table
 // (S*S + 64C float2) and, for BT, the c table (N*S + 64C floats).  PH: the
 // phase-timing instantiation (run_trajectory).
@@ -21,8 +22,10 @@ kmc_batch_fp32_kernel(BatchParams p) {
   // from them) in scalar registers with scalar branches
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
-  float* evc = reinterpret_cast<float*>(smem) + wave * 2 * PMAX;
-  float* elw = reinterpret_cast<float*>(smem) + W * 2 * PMAX;
+  const int sb = store_wave_bytes(p.store_slots);
+  unsigned char* store = sb ? smem + wave * sb : nullptr;
+  float* evc = reinterpret_cast<float*>(smem + W * sb) + wave * 2 * PMAX;
+  float* elw = reinterpret_cast<float*>(smem + W * sb) + W * 2 * PMAX;
   __shared__ int s_group;
   __shared__ int s_layout;
   if (threadIdx.x == 0) s_layout = -1;
@@ -34,15 +37,20 @@ kmc_batch_fp32_kernel(BatchParams p) {
     if (g >= p.ngroups) break;
     const GroupDev gd = p.groups[g];
     const LayoutDev L = p.layouts[gd.layout];
-    float2* tab = reinterpret_cast<float2*>(elw + elw_lds_floats(L.P, C));
-    float* ctab = reinterpret_cast<float*>(tab + L.S * L.S + 64 * C);
+    // a network of S <= 64 sites runs in one-chunk code (its tables laid out
+    // for one chunk): the bits of any chunk count, with the state store
+    const bool one = C > 1 && L.S <= 64;
+    const int Ce = one ? 1 : C;
+    float2* tab = reinterpret_cast<float2*>(elw + elw_lds_floats(L.P, Ce));
+    float* ctab = reinterpret_cast<float*>(tab + L.S * L.S + 64 * Ce);
     if (gd.layout != __builtin_amdgcn_readfirstlane(s_layout)) {
-      load_elw<C>(elw, L, threadIdx.x, 64 * W);
-      const int n = L.S * L.S + 64 * C;
+      if (one) load_elw<1>(elw, L, threadIdx.x, 64 * W);
+      else load_elw<C>(elw, L, threadIdx.x, 64 * W);
+      const int n = L.S * L.S + 64 * Ce;
       const float2* src = BT ? L.tabB : L.tab32;
       for (int k = threadIdx.x; k < n; k += 64 * W) tab[k] = src[k];
       if (BT) {
-        const int nc = L.N * L.S + 64 * C;
+        const int nc = L.N * L.S + 64 * Ce;
         for (int k = threadIdx.x; k < nc; k += 64 * W) ctab[k] = L.ccol[k];
       }
       __syncthreads();
@@ -50,21 +58,30 @@ kmc_batch_fp32_kernel(BatchParams p) {
     }
     __syncthreads();
     if (wave < gd.count) {
-      const TabPolicy<C, BT> pol = TabPolicy<C, BT>::make(tab, ctab, elw, evc, L.S, lane);
-      run_trajectory<C, REC, TabPolicy<C, BT>, 1, PRIO, PH>(p, gd.first + wave, L,
-                                                                        pol, lane);
+      if (one) {
+        const TabPolicy<1, BT> pol = TabPolicy<1, BT>::make(tab, ctab, elw, evc, L.S, lane);
+        run_trajectory<1, REC, TabPolicy<1, BT>, 1, PRIO, PH>(p, gd.first + wave, L, pol, lane,
+                                                              Team<1, 1>{}, store);
+      } else {
+        const TabPolicy<C, BT> pol = TabPolicy<C, BT>::make(tab, ctab, elw, evc, L.S, lane);
+        run_trajectory<C, REC, TabPolicy<C, BT>, 1, PRIO, PH>(p, gd.first + wave, L, pol, lane,
+                                                              Team<1, C>{}, store);
+      }
     }
   }
 }
 
 // Positions policy: every wave pulls its own groups and keeps only its layout's
 // site coordinates in LDS (at most 64*C float4), so a workgroup mixes layouts.
+// The waves' state stores follow the coordinates (W * store_wave_bytes).
 template <int C, int W, bool REC>
 __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(C == 1 ? 4 : 1)))
 kmc_batch_pos_kernel(BatchParams p) {
   extern __shared__ __align__(16) unsigned char smem[];
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   float4* pos = reinterpret_cast<float4*>(smem) + wave * (64 * C);
+  const int sb = store_wave_bytes(p.store_slots);
+  unsigned char* store = sb ? smem + W * 64 * C * sizeof(float4) + wave * sb : nullptr;
   for (;;) {
     int g = 0;
     if (lane == 0) g = atomicAdd(p.queue, 1);
@@ -80,12 +97,14 @@ kmc_batch_pos_kernel(BatchParams p) {
       PosPolicy<1> pol;
       pol.init(pos, L, lane);
       for (int i = 0; i < gd.count; ++i)
-        run_trajectory<1, REC, PosPolicy<1>, 1, true>(p, gd.first + i, L, pol, lane);
+        run_trajectory<1, REC, PosPolicy<1>, 1, true>(p, gd.first + i, L, pol, lane,
+                                                      Team<1, 1>{}, store);
     } else {
       PosPolicy<C> pol;
       pol.init(pos, L, lane);
       for (int i = 0; i < gd.count; ++i)
-        run_trajectory<C, REC, PosPolicy<C>, 1, true>(p, gd.first + i, L, pol, lane);
+        run_trajectory<C, REC, PosPolicy<C>, 1, true>(p, gd.first + i, L, pol, lane,
+                                                      Team<1, C>{}, store);
     }
     wave_lds_sync();
   }
@@ -148,5 +167,9 @@ size_t batch_lds_bytes(bool bt, int C, int W, int S, int N, int P) {
 }
 
 size_t pos_lds_bytes(int C, int W) { return (size_t)W * 64 * C * sizeof(float4); }
+
+// LDS bytes of the W waves' state stores of `slots` slots each (they come before
+// the table kernels' other LDS and after the positions kernels' coordinates).
+size_t store_lds_bytes(int W, int slots) { return (size_t)W * store_wave_bytes(slots); }
 
 }  // namespace kmc

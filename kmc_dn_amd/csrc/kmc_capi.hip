@@ -35,6 +35,7 @@ __global__ void kmc_eset_scatter_kernel(const EsetDev*, const int32_t*, const in
 size_t plain_lds_bytes(int S, int N, bool stage);
 size_t batch_lds_bytes(bool bt, int C, int W, int S, int N, int P);
 size_t pos_lds_bytes(int C, int W);
+size_t store_lds_bytes(int W, int slots);
 template <int C, int T, bool REC, int FORM, bool PH>
 __global__ void kmc_team_kernel(BatchParams);
 size_t team_lds_bytes(int form, int C, int T, int S, int N, int P);
@@ -910,6 +911,31 @@ int setup_launch(Batch* B, const kmc_run_opts& o, int form, int W, int team, int
   int per_cu = 0;
   HIPOK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k, 64 * W, lds));
   if (per_cu <= 0) return fail(KMC_ERR_LIMIT, "kernel does not fit on a CU (LDS %zu)", lds);
+  // The state stores of one-chunk solo kernels: as many slots as leave the
+  // resident workgroups per CU unchanged.  They change no result.  (Kernels
+  // over several site chunks run their S <= 64 networks in one-chunk code
+  // without a store: their LDS would take room that a plan's team
+  // workgroups, placed beside them, need.)
+  int slots = 0;
+  if (team <= 1 && C == 1 && o.state_store >= 0) {
+    const int want = o.state_store > 0 ? std::min(o.state_store, STORE_MAX_SLOTS) : STORE_MAX_SLOTS;
+    for (int v = want; v > 0 && !slots; --v) {
+      const size_t l2 = lds + store_lds_bytes(W, v);
+      if (l2 > lds_cap) continue;
+      // (refused when l2 and the kernel's static LDS exceed a workgroup's share)
+      if (hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)l2) !=
+          hipSuccess) {
+        (void)hipGetLastError();
+        continue;
+      }
+      int pc = 0;
+      HIPOK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&pc, (const void*)k, 64 * W, l2));
+      if (pc >= per_cu) {
+        slots = v;
+        lds = l2;
+      }
+    }
+  }
   // The graded issue priority of the hop pays while every group is resident at
   // once (4096 trajectories: +12 %); with groups queued behind full SIMDs the
   // variant without it is 2 % faster (65,536-trajectory IV sweep).
@@ -939,6 +965,7 @@ int setup_launch(Batch* B, const kmc_run_opts& o, int form, int W, int team, int
   p.resume = o.resume ? 1 : 0;
   p.phases = phases;
   p.time_limit = o.time_limit_us > 0 ? (int64_t)o.time_limit_us * E.wall_khz / 1000 : 0;
+  p.store_slots = slots;
   L.k = k;
   L.grid = grid;
   L.block = 64 * W;
@@ -1174,6 +1201,8 @@ int batch_run_locked(int32_t id, const kmc_run_opts* opts) {
   if (st) return st;
   hipStream_t s = E.stream;
   if (o.time_limit_us < 0) return fail(KMC_ERR_ARGS, "negative time limit %d us", o.time_limit_us);
+  if (o.state_store < -1 || o.state_store > STORE_MAX_SLOTS)
+    return fail(KMC_ERR_ARGS, "state_store %d outside [-1, %d]", o.state_store, STORE_MAX_SLOTS);
   if (o.precision == KMC_FP64_DENSE) {
     if (o.resume) return fail(KMC_ERR_UNSUPPORTED, "resume runs use the FP32 batched kernel");
     if (o.time_limit_us)

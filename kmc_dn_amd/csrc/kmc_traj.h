@@ -33,12 +33,16 @@
 //  * Event pick: FP64 wave prefix sum of R (DPP), Philox event uniform * total,
 //    ballot for the first lane whose prefix exceeds it, then one lane-parallel
 //    pass over that lane's pairs (FP32 DPP scan) for the partner site.
-//  * Hop: occupation bits flip in SGPRs; energies get the O(N) rank-1 Coulomb
-//    update of makeJump (simulation.go:107-130); electrode tallies stay in the
-//    electrode lanes' registers; the clock is accumulated in FP64.
+//  * Hop: occupation bits flip in SGPRs; electrode tallies stay in the
+//    electrode lanes' registers; the clock is accumulated in FP64.  A new
+//    state's energies get O(N) Coulomb column updates (makeJump,
+//    simulation.go:107-130): in one-chunk solo code the last FP64 recompute's
+//    energies plus one column per acceptor whose occupation changed since, in
+//    ascending site order (a function of the occupation alone); elsewhere the
+//    previous state's plus the rank-1 update of the hop.
 //  * At every multiple of `recompute_interval` hops the energies are rebuilt in
-//    FP64 from E_constant and the I_0*R/d table, cancelling FP32 drift of the
-//    updates (the Go reference never re-synchronises).
+//    FP64 from E_constant and the I_0*R/d table (the Go reference never
+//    re-synchronises).
 //  * Uniforms come from one Philox block per lane every 64 hops (lane k holds
 //    hop h0+k), so the RNG costs two readlanes per hop.
 //  * A trajectory runs the absolute hop range [h_begin, h_end) of its budget
@@ -77,10 +81,28 @@ __device__ __forceinline__ void recompute_energies(float (&se)[C], const uint64_
     const int s = lane + 64 * c;
     float v = 0.f;
     if (s < N) {
+      // the sum over the empty acceptors in ascending order; the terms of eight
+      // acceptors are loaded together (one memory round trip per eight; an
+      // index past the last loads the last's term and adds nothing)
+      const double e = es.E[s];
+      const double* col = L.coul64 + s;
       double acc = 0.0;
-      for (int j = 0; j < N; ++j)
-        if (!((occ[j >> 6] >> (j & 63)) & 1ull) && j != s) acc += L.coul64[j * N + s];
-      v = (float)((es.E[s] - acc) * sc);
+      if constexpr (C == 1) {
+        for (int j0 = 0; j0 < N; j0 += 8) {
+          double t[8];
+#pragma unroll
+          for (int g = 0; g < 8; ++g) t[g] = col[min(j0 + g, N - 1) * N];
+#pragma unroll
+          for (int g = 0; g < 8; ++g) {
+            const int j = j0 + g;
+            acc += (j < N && !((occ[0] >> (j & 63)) & 1ull) && j != s) ? t[g] : 0.0;
+          }
+        }
+      } else {
+        for (int j = 0; j < N; ++j)
+          if (!((occ[j >> 6] >> (j & 63)) & 1ull) && j != s) acc += col[j * N];
+      }
+      v = (float)((e - acc) * sc);
     } else if (s < S) {
       v = (float)(es.V[s - N] * sc);
     }
@@ -132,7 +154,7 @@ __device__ __forceinline__ float min_f32(float a, float b) {
 // Pair-term policies.  For a uniform member u of the walked set and the lane
 // sites s = lane + 64c, pair(pre(u), c, diff) is log2 of the rate of the pair
 // (src, dst) with e~_src - e~_dst = diff; coul(u, c) is c(s, u) for
-// the rank-1 energy update (u an acceptor).  prefetch(u) issues the loads that
+// the energy update (u an acceptor).  prefetch(u) issues the loads that
 // pair() needs, so four members can be in flight together.
 // ---------------------------------------------------------------------------
 template <int C, bool BT>
@@ -464,12 +486,11 @@ __device__ __forceinline__ uint64_t uniform_u64(uint64_t x) {
 // state before it.  Every hop changes the occupation, so the next state is
 // either the one before (a hit: nothing is evaluated, the cached row sums,
 // FP64 prefix, energies and, for the same selected row, the partner prefix
-// serve the hop) or new (a miss: its energies are the previous state's plus
-// the rank-1 update of the hop that left it, and its row sums are evaluated
-// into the older entry).  The entries swap roles on every hop; the loop is
-// unrolled by two hops, one for each assignment of the roles, so the entries
-// sit in fixed registers and a hit moves no data.  The entries are emptied at every FP64
-// recompute, so a trajectory's results depend on its own history only, and
+// serve the hop) or new (a miss: its energies are formed from the last
+// recompute's, and its row sums are evaluated into the older entry).  The
+// entries swap roles on every hop; the loop is unrolled by two hops, one for
+// each assignment of the roles, so the entries sit in fixed registers and a
+// hit moves no data.  The entries are emptied at every FP64 recompute, so
 // ranges that start at multiples of the recompute interval continue exactly.
 // The Go library keeps a cache keyed by occupation too
 // (simulation.go:254-296); at low bias a trajectory mostly moves one hole
@@ -480,9 +501,21 @@ __device__ __forceinline__ uint64_t uniform_u64(uint64_t x) {
 // boundaries of every hop (FP32_PHASES), summed into p.phases by the first wave
 // of each trajectory.  The reads wait for the wave's outstanding LDS and scalar
 // loads, so its hops are somewhat slower than the others'.
+//
+// A one-chunk solo trajectory also keeps the states its two entries evict in a
+// state store of p.store_slots slots in LDS (`store`, this wave's; null: none):
+// a new state found there is swapped back into the entry instead of evaluated.
+// The energies of a state are a function of its occupation alone between two
+// recomputes (the recompute's energies plus the Coulomb columns of the
+// acceptors whose occupation differs from the recompute's, added in ascending
+// site order), so a stored entry holds the bits an evaluation would give: the
+// store changes no result, only how often a state is evaluated.  Trajectories
+// that hop among three or four states (at low bias, the slowest of an
+// ensemble) miss the two entries on several per cent of their hops.
 template <int C, bool REC, class Pol, int T = 1, bool PRIO = true, bool PH = false>
 __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, const Pol& pol,
-                               int lane, const Team<T, C>& team = Team<T, C>{}) {
+                               int lane, const Team<T, C>& team = Team<T, C>{},
+                               unsigned char* store = nullptr) {
 
   const int64_t t_start = wall_clock64();
   const TrajDev td = p.trajs[t];
@@ -522,6 +555,15 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
   };
   int32_t status = __builtin_amdgcn_readfirstlane(cont ? o->status : (int32_t)ST_OK);
   recompute_energies<C>(se, occ, L, es, lane);
+  // the last recompute's energies and occupation: every state's energies derive
+  // from them
+  float base_se[C];
+  uint64_t base_occ[C];
+#pragma unroll
+  for (int c = 0; c < C; ++c) {
+    base_se[c] = se[c];
+    base_occ[c] = occ[c];
+  }
   uint64_t accm[C], elm[C];   // lanes of chunk c holding acceptors / electrodes
 #pragma unroll
   for (int c = 0; c < C; ++c) {
@@ -620,9 +662,38 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
   E0.val = E1.val = 0;
   E0.plast = E1.plast = -1;
   bool cur_ok = false;   // the current state's entry holds it
-  // the last hop's ends and whether its energy update is still to be applied
+  // Energies of a new state.  One-chunk solo code: the recompute's plus the
+  // columns of the acceptors changed since, a function of the occupation alone
+  // (what lets the state store restore a state's bits).  Teams and kernels
+  // over several site chunks: the previous state's plus the rank-1 update of
+  // the hop that left it (two columns; the miss-heavy trajectories these run
+  // change many sites between recomputes); psrc, pdst: that hop's ends, pend:
+  // its update is still to be applied.
+  constexpr bool kPathFree = C == 1 && T == 1;
   int psrc = 0, pdst = 0;
   bool pend = false;
+  // the state store: slots in use (bit v: slot v) and the slot the next evicted
+  // state without a slot of its own takes (round robin)
+  constexpr bool kStore = kPathFree;
+  const int nslots = kStore && store ? p.store_slots : 0;
+  unsigned svalid = 0;
+  int snext = 0;
+  uint64_t* const skey = reinterpret_cast<uint64_t*>(store);
+  uint64_t* const snzr = skey + STORE_MAX_SLOTS;
+  auto slot_inc = [&](int v) {
+    return reinterpret_cast<double*>(store + STORE_HEADER_BYTES + v * STORE_SLOT_BYTES);
+  };
+  // entry W's state into slot v (W valid)
+  auto store_put = [&](int v, const StateEntry<C>& W) __attribute__((always_inline)) {
+    double* inc = slot_inc(v);
+    inc[lane] = W.inc[0];
+    inc[64 + lane] = W.exc[0];
+    reinterpret_cast<float*>(inc + 128)[lane] = W.se[0];
+    if (lane == 0) {
+      skey[v] = W.key[0];
+      snzr[v] = W.nzr[0];
+    }
+  };
 
   // The current state's row sums, FP64 prefix and total into W, from the
   // energies se; false: no rate is positive and finite.
@@ -831,7 +902,7 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
       tr[dst * S + src] -= 1.0;
     }
     // perform hop: occupation (acceptors live in the words of the C chunks),
-    // tallies; the energy update is left to a miss
+    // tallies; a new state's energies are formed when it is evaluated
     const bool src_acc = src < N, dst_acc = dst < N;
 #pragma unroll
     for (int c = 0; c < C; ++c) {
@@ -891,10 +962,17 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
     }
     if (h == next_rc) {
       recompute_energies<C>(se, occ, L, es, lane);
+#pragma unroll
+      for (int c = 0; c < C; ++c) {
+        base_se[c] = se[c];
+        base_occ[c] = occ[c];
+      }
       next_rc = next_multiple(h);
       pend = false;
       cur_ok = false;
       E0.val = E1.val = 0;
+      svalid = 0;
+      snext = 0;
     }
     special = next_special(h, false);
     h_lim = special < h_stop ? special : h_stop;
@@ -910,29 +988,110 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
       if (__builtin_expect(h == special, 0) && !special_work()) return false;
     }
     if (!cur_ok) {
-      // a new state: its energies are the previous state's plus the last hop's
-      // rank-1 update (both Coulomb columns loaded together; an electrode end
-      // contributes 0), or the recomputed ones
-      if (pend) {
-        const bool src_acc = psrc < N, dst_acc = pdst < N;
-        float cs[C], cd[C];
+      // a new state (F's old state, two hops back, leaves the entries)
+      B.plast = -1;   // B's hop back led into the state that F held
+      bool restored = false;
+      if constexpr (kStore) {
+        if (nslots > 0) {
+          mark(1);   // the time since the last hop: a miss's, whether restored or evaluated
+          // the slots' keys by broadcast reads (a uniform address: no lane
+          // address to keep in a register through the hop loop)
+          unsigned found = 0;
 #pragma unroll
-        for (int c = 0; c < C; ++c) {
-          if constexpr (Pol::kZeroColumnN) {   // column N holds c = 0: no select per lane
-            cs[c] = (c == 0 || 64 * c < S) ? pol.coul(min((unsigned)psrc, (unsigned)N), c) : 0.f;
-            cd[c] = (c == 0 || 64 * c < S) ? pol.coul(min((unsigned)pdst, (unsigned)N), c) : 0.f;
-          } else {
-            cs[c] = (c == 0 || 64 * c < S) && src_acc ? pol.coul(psrc, c) : 0.f;
-            cd[c] = (c == 0 || 64 * c < S) && dst_acc ? pol.coul(pdst, c) : 0.f;
+          for (int v = 0; v < STORE_MAX_SLOTS; ++v)
+            found |= (uniform_u64(skey[v]) == occ[0] ? 1u : 0u) << v;
+          found &= svalid;
+          if (found) {
+            // swap: the stored state into F, F's old state into its slot
+            const int v = __builtin_ctz(found);
+            double* inc = slot_inc(v);
+            const double si = inc[lane], sx = inc[64 + lane];
+            const float sv = reinterpret_cast<const float*>(inc + 128)[lane];
+            const uint64_t nz = snzr[v];
+            if (F.val) store_put(v, F);
+            else svalid &= ~(1u << v);
+            int nocc = popcount64(occ[0] & accm[0]);
+            F.inc[0] = si;
+            F.exc[0] = sx;
+            F.se[0] = sv;
+            F.nzr[0] = in_vgpr(nz);
+            F.tot = in_vgpr(readlane_f64(si, 63));
+            F.rtot = __builtin_amdgcn_rcpf((float)F.tot);
+            F.key[0] = occ[0];
+            F.udst = N - nocc <= nocc;
+            F.psel = -2;
+            F.plast = -1;
+            F.val = 1;
+            restored = true;
+            if constexpr (PH) ++ph[10];
+            mark(2);
+          } else if (F.val) {
+            store_put(snext, F);
+            svalid |= 1u << snext;
+            snext = snext + 1 == nslots ? 0 : snext + 1;
           }
         }
-#pragma unroll
-        for (int c = 0; c < C; ++c) se[c] = (B.se[c] - cs[c]) + cd[c];
       }
-      B.plast = -1;   // B's hop back led into the state that F held
-      if (!evaluate(F)) {
-        status = ST_ZERO_RATE;
-        return false;
+      if (!restored) {
+        if constexpr (kPathFree) {
+        // its energies: the recompute's plus the Coulomb columns of the
+        // acceptors whose occupation differs from the recompute's (+ where
+        // now occupied, - where now empty), added in ascending site order;
+        // the columns of up to G sites are loaded together (a site past the
+        // last loads the first's column and adds nothing)
+        constexpr int G = 8;
+#pragma unroll
+        for (int c = 0; c < C; ++c) se[c] = base_se[c];
+#pragma unroll
+        for (int w = 0; w < C; ++w) {
+          uint64_t dm = (occ[w] ^ base_occ[w]) & accm[w];
+          while (dm) {
+            int js[G];
+            const int j0 = __builtin_ctzll(dm);
+#pragma unroll
+            for (int g = 0; g < G; ++g) {
+              js[g] = dm ? __builtin_ctzll(dm) : -1;
+              dm &= dm - 1;
+            }
+            float cv[G][C];
+#pragma unroll
+            for (int g = 0; g < G; ++g)
+#pragma unroll
+              for (int c = 0; c < C; ++c)
+                cv[g][c] = (c == 0 || 64 * c < S) ? pol.coul(64 * w + (js[g] < 0 ? j0 : js[g]), c)
+                                                  : 0.f;
+#pragma unroll
+            for (int g = 0; g < G; ++g) {
+              if (js[g] < 0) break;
+              const bool on = (occ[w] >> js[g]) & 1ull;
+#pragma unroll
+              for (int c = 0; c < C; ++c) se[c] += on ? cv[g][c] : -cv[g][c];
+            }
+          }
+        }
+        } else if (pend) {
+          // the previous state's energies plus the last hop's rank-1 update
+          // (both Coulomb columns loaded together; an electrode end
+          // contributes 0); after a recompute, the recomputed ones
+          const bool src_acc = psrc < N, dst_acc = pdst < N;
+          float cs[C], cd[C];
+#pragma unroll
+          for (int c = 0; c < C; ++c) {
+            if constexpr (Pol::kZeroColumnN) {   // column N holds c = 0: no select per lane
+              cs[c] = (c == 0 || 64 * c < S) ? pol.coul(min((unsigned)psrc, (unsigned)N), c) : 0.f;
+              cd[c] = (c == 0 || 64 * c < S) ? pol.coul(min((unsigned)pdst, (unsigned)N), c) : 0.f;
+            } else {
+              cs[c] = (c == 0 || 64 * c < S) && src_acc ? pol.coul(psrc, c) : 0.f;
+              cd[c] = (c == 0 || 64 * c < S) && dst_acc ? pol.coul(pdst, c) : 0.f;
+            }
+          }
+#pragma unroll
+          for (int c = 0; c < C; ++c) se[c] = (B.se[c] - cs[c]) + cd[c];
+        }
+        if (!evaluate(F)) {
+          status = ST_ZERO_RATE;
+          return false;
+        }
       }
       cur_ok = true;
     } else if constexpr (PH) {

@@ -126,7 +126,22 @@ struct BatchParams {
                                 // instantiations only), or null
   int64_t time_limit;           // constant-rate clock ticks a solo trajectory runs before
                                 // it stops at its next recompute point; 0: none
+  int32_t store_slots;          // slots of each wave's LDS state store (one-chunk solo
+                                // trajectories; 0: none)
+  int32_t pad2;
 };
+
+// The state store of a one-chunk solo trajectory: states evicted from its two
+// register entries, kept in LDS per wave.  Layout: STORE_MAX_SLOTS occupation
+// keys (u64), then STORE_MAX_SLOTS nonzero-row lane masks (u64), then per slot
+// the FP64 inclusive and exclusive row prefixes and the FP32 energies of the
+// 64 lanes.
+constexpr int STORE_MAX_SLOTS = 4;
+constexpr int STORE_HEADER_BYTES = 2 * STORE_MAX_SLOTS * 8;
+constexpr int STORE_SLOT_BYTES = 64 * (8 + 8 + 4);
+constexpr int store_wave_bytes(int slots) {
+  return slots > 0 ? STORE_HEADER_BYTES + slots * STORE_SLOT_BYTES : 0;
+}
 
 // Hop phases of the FP32 batched kernels timed by their phase-timing
 // instantiations (shader cycles of each trajectory's first wave, summed over
@@ -136,8 +151,9 @@ struct BatchParams {
 // miss, 3 clock and event selection, 4 partner selection, 5 hop (occupation,
 // tallies) of the general step, 6 fast hops and leaps; slot 7 counts the hops
 // from a cached state, slot 8 those whose partner prefix came from the state's
-// entry, slot 9 those taken by leaps.
-constexpr int FP32_PHASES = 10;
+// entry, slot 9 those taken by leaps, slot 10 the new states restored from the
+// LDS state store instead of evaluated.
+constexpr int FP32_PHASES = 11;
 
 struct PlainParams {
   const TrajDev* trajs;

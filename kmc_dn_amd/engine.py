@@ -161,11 +161,12 @@ class Batch:
             recompute_interval: int = 0, record: bool = False, post_hop_record: bool = False,
             grid_limit: int = 0, use_graph: bool = False, traffic: bool = False,
             table_form: int = 0, resume: bool = False, team: int = 1,
-            time_limit_us: int = 0) -> None:
+            time_limit_us: int = 0, state_store: int = 0) -> None:
         rec = (1 if record else 0) | (2 if traffic else 0)
         o = L.kmc_run_opts(int(precision), int(waves_per_group), int(recompute_interval),
                            rec, int(post_hop_record), int(grid_limit), int(use_graph),
-                           int(table_form), int(bool(resume)), int(team), int(time_limit_us))
+                           int(table_form), int(bool(resume)), int(team), int(time_limit_us),
+                           int(state_store))
         L.check(L.load().kmc_batch_run(self.id, C.byref(o)))
         self.recorded = bool(record)
 

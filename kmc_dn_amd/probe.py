@@ -105,13 +105,15 @@ def _fp32_phase_run(b, run: dict, hops_total: int) -> dict:
         b.run(**run)
         ms = b.sync()
     o = _lib.kmc_run_opts(_lib.FP32_BATCHED, int(run.get("waves_per_group", 0)), 0, 0, 0, 0, 0,
-                          int(run.get("table_form", 0)), 0, int(run.get("team", 1)))
-    cyc = (C.c_uint64 * 10)()
-    _lib.check(_lib.load().kmc_probe_fp32_phases(b.id, C.byref(o), cyc, 10))
+                          int(run.get("table_form", 0)), 0, int(run.get("team", 1)), 0,
+                          int(run.get("state_store", 0)))
+    cyc = (C.c_uint64 * 11)()
+    _lib.check(_lib.load().kmc_probe_fp32_phases(b.id, C.byref(o), cyc, 11))
     per = [cyc[k] / hops_total for k in range(len(FP32_PHASES))]
     return {"kernel_ms": round(ms, 3), "hit_rate": round(cyc[7] / hops_total, 5),
             "partner_hit_rate": round(cyc[8] / hops_total, 5),
             "leap_rate": round(cyc[9] / hops_total, 5),
+            "store_restore_rate": round(cyc[10] / hops_total, 6),
             "cycles_per_hop": dict(zip(FP32_PHASES, [round(x, 1) for x in per])),
             "cycles_per_hop_total": round(sum(per), 1)}
 

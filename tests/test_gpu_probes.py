@@ -63,8 +63,8 @@ def test_fp32_phase_instantiation_matches(gpu_engine, N, P, form, team):
         r0 = b.results()
         o = _lib.kmc_run_opts(_lib.FP32_BATCHED, run["waves_per_group"], 0, 0, 0, 0, 0, form, 0,
                               team)
-        cyc = (C.c_uint64 * 10)()
-        _lib.check(_lib.load().kmc_probe_fp32_phases(b.id, C.byref(o), cyc, 10))
+        cyc = (C.c_uint64 * 11)()
+        _lib.check(_lib.load().kmc_probe_fp32_phases(b.id, C.byref(o), cyc, 11))
         _same(r0, b.results())
         total = K * hops
         assert 0 < cyc[7] <= total and 0 < cyc[8] <= total and cyc[9] <= cyc[8]
@@ -87,12 +87,12 @@ def test_fp32_phase_probe_rejects_shapes_it_has_no_kernel_for(gpu_engine):
     try:
         for form, w, team in [(3, 4, 1), (1, 2, 1)]:
             o = _lib.kmc_run_opts(_lib.FP32_BATCHED, w, 0, 0, 0, 0, 0, form, 0, team)
-            cyc = (C.c_uint64 * 10)()
-            assert _lib.load().kmc_probe_fp32_phases(b.id, C.byref(o), cyc, 10) != 0
+            cyc = (C.c_uint64 * 11)()
+            assert _lib.load().kmc_probe_fp32_phases(b.id, C.byref(o), cyc, 11) != 0
         o = _lib.kmc_run_opts(_lib.FP32_BATCHED, 4, 0, 1, 0, 0, 0, 1, 0, 1)   # records
-        assert _lib.load().kmc_probe_fp32_phases(b.id, C.byref(o), cyc, 10) != 0
+        assert _lib.load().kmc_probe_fp32_phases(b.id, C.byref(o), cyc, 11) != 0
         o = _lib.kmc_run_opts(_lib.FP32_BATCHED, 4, 0, 0, 0, 0, 0, 1, 0, 1)
-        assert _lib.load().kmc_probe_fp32_phases(b.id, C.byref(o), cyc, 9) != 0   # slots
+        assert _lib.load().kmc_probe_fp32_phases(b.id, C.byref(o), cyc, 10) != 0   # slots
     finally:
         b.close()
         lay.close()
