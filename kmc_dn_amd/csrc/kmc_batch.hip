@@ -60,8 +60,8 @@ kmc_batch_fp32_kernel(BatchParams p) {
     if (wave < gd.count) {
       if (one) {
         const TabPolicy<1, BT> pol = TabPolicy<1, BT>::make(tab, ctab, elw, evc, L.S, lane);
-        run_trajectory<1, REC, TabPolicy<1, BT>, 1, PRIO, PH>(p, gd.first + wave, L, pol, lane,
-                                                              Team<1, 1>{}, store);
+        run_trajectory<1, REC, TabPolicy<1, BT>, 1, PRIO, PH, false>(p, gd.first + wave, L, pol,
+                                                                     lane, Team<1, 1>{}, store);
       } else {
         const TabPolicy<C, BT> pol = TabPolicy<C, BT>::make(tab, ctab, elw, evc, L.S, lane);
         run_trajectory<C, REC, TabPolicy<C, BT>, 1, PRIO, PH>(p, gd.first + wave, L, pol, lane,
@@ -97,8 +97,8 @@ kmc_batch_pos_kernel(BatchParams p) {
       PosPolicy<1> pol;
       pol.init(pos, L, lane);
       for (int i = 0; i < gd.count; ++i)
-        run_trajectory<1, REC, PosPolicy<1>, 1, true>(p, gd.first + i, L, pol, lane,
-                                                      Team<1, 1>{}, store);
+        run_trajectory<1, REC, PosPolicy<1>, 1, true, false, false>(p, gd.first + i, L, pol, lane,
+                                                                    Team<1, 1>{}, store);
     } else {
       PosPolicy<C> pol;
       pol.init(pos, L, lane);

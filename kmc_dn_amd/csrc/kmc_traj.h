@@ -512,7 +512,11 @@ __device__ __forceinline__ uint64_t uniform_u64(uint64_t x) {
 // store changes no result, only how often a state is evaluated.  Trajectories
 // that hop among three or four states (at low bias, the slowest of an
 // ensemble) miss the two entries on several per cent of their hops.
-template <int C, bool REC, class Pol, int T = 1, bool PRIO = true, bool PH = false>
+// XF: fast hops also take the hops into an uncached row or a new state (one-chunk
+// kernels; kernels over several site chunks keep the shorter fast hop for their
+// one-chunk code too, which holds their registers to the occupancy they run at).
+template <int C, bool REC, class Pol, int T = 1, bool PRIO = true, bool PH = false,
+          bool XF = (C == 1)>
 __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, const Pol& pol,
                                int lane, const Team<T, C>& team = Team<T, C>{},
                                unsigned char* store = nullptr) {
@@ -662,6 +666,7 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
   E0.val = E1.val = 0;
   E0.plast = E1.plast = -1;
   bool cur_ok = false;   // the current state's entry holds it
+  bool fresh = false;    // a fast hop has just taken a hop into a new state
   // Energies of a new state.  One-chunk solo code: the recompute's plus the
   // columns of the acceptors changed since, a function of the occupation alone
   // (what lets the state store restore a state's bits).  Teams and kernels
@@ -802,6 +807,37 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
     return total > 0.0 && total < INFINITY;
   };
 
+  // The partner prefix of row `sel` of F's state into F: the prefix of the
+  // row's rates over the lanes in site order, and the lanes whose rate is
+  // nonzero (a lane-parallel pass over the row).
+  auto partner_pass = [&](StateEntry<C>& F, int sel) __attribute__((always_inline)) {
+    uint64_t inu[C];   // partners: targets of a source row, sources of a target row
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      const uint64_t o = occ[c] & accm[c];
+      inu[c] = (F.udst ? accm[c] ^ o : o) | elm[c];
+    }
+    const float s_sel = pick_lane_value<C>(F.se, sel);
+    // sign of the lane-parallel difference: a scalar constant
+    const float sg = __int_as_float(F.udst ? (int)0xbf800000 : (int)0x3f800000);
+    const typename Pol::Pre pre = pol.prefetch(sel);
+    float run = 0.f;
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      F.pcum[c] = 0.f;
+      F.pnz[c] = 0;
+      if (c > 0 && 64 * c >= S) continue;
+      // udst ? s_sel - se : se - s_sel (the sign is exact)
+      float r = fast_exp2(pol.pair(pre, c, F.se[c] - s_sel, sg));
+      r = __builtin_amdgcn_inverse_ballot_w64(inu[c]) ? r : 0.f;
+      F.pcum[c] = wave_scan_f32(r) + run;
+      F.pnz[c] = ballot(r > 0.f);
+      run = readlane_f32(F.pcum[c], 63);
+    }
+    F.psel = sel;
+    F.plast = -1;
+  };
+
   // One hop from the current state F (its entry valid); B: the other entry.
   // Returns 1 when the state it leads to is B's, 0 when that state is new,
   // -1 when the trajectory stops.
@@ -844,31 +880,7 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
     // the lanes whose rate is nonzero), from the entry when it was formed for
     // the same row, else by a lane-parallel pass over the row
     if (sel != F.psel) {
-      uint64_t inu[C];   // partners: targets of a source row, sources of a target row
-#pragma unroll
-      for (int c = 0; c < C; ++c) {
-        const uint64_t o = occ[c] & accm[c];
-        inu[c] = (F.udst ? accm[c] ^ o : o) | elm[c];
-      }
-      const float s_sel = pick_lane_value<C>(F.se, sel);
-      // sign of the lane-parallel difference: a scalar constant
-      const float sg = __int_as_float(F.udst ? (int)0xbf800000 : (int)0x3f800000);
-      const typename Pol::Pre pre = pol.prefetch(sel);
-      float run = 0.f;
-#pragma unroll
-      for (int c = 0; c < C; ++c) {
-        F.pcum[c] = 0.f;
-        F.pnz[c] = 0;
-        if (c > 0 && 64 * c >= S) continue;
-        // udst ? s_sel - se : se - s_sel (the sign is exact)
-        float r = fast_exp2(pol.pair(pre, c, F.se[c] - s_sel, sg));
-        r = __builtin_amdgcn_inverse_ballot_w64(inu[c]) ? r : 0.f;
-        F.pcum[c] = wave_scan_f32(r) + run;
-        F.pnz[c] = ballot(r > 0.f);
-        run = readlane_f32(F.pcum[c], 63);
-      }
-      F.psel = sel;
-      F.plast = -1;
+      partner_pass(F, sel);
     } else if constexpr (PH) {
       ++ph[8];
     }
@@ -1108,13 +1120,15 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
     mark(5);
     return h < h_stop;
   };
-  // A hit hop in its shortest form: the current state's entry F is valid, its
-  // partner prefix was formed for the row the event falls in and the hop leads
-  // back into the state of the other entry B.  The arithmetic is the general
-  // hop's (same target, remainder, partner and clock bits); only the
-  // bookkeeping that such a hop cannot need is left out, so the scalar unit
-  // that the waves of a CU share issues a third of the general path's
-  // instructions.  Anything else (an uncached row, a new state, the end of the
+  // A hop from a cached state in its shortest form: the current state's entry
+  // F is valid.  The arithmetic is the general hop's (same target, remainder,
+  // partner prefix, partner and clock bits); only the bookkeeping that such a
+  // hop cannot need is left out, so the scalar unit that the waves of a CU
+  // share issues a third of the general path's instructions.  A row whose
+  // partner prefix is not cached gets it here; a hop back into the state of the
+  // other entry B returns true, a hop into a new state is taken and returns
+  // false with `fresh` set (the caller evaluates the state).  The rare rest (a
+  // target rounded onto the total, a row without a partner, the end of the
   // range, uniforms or a recompute due) returns false with nothing changed, and
   // the general step takes the hop.  Kernels that record take the general step
   // on every hop.
@@ -1148,7 +1162,8 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
         int n = lowest_bit(~run);
         const int nmax = 64 - hl < h_lim - h ? 64 - hl : h_lim - h;
         if ((unsigned)n > (unsigned)nmax) n = nmax;
-        if (n == 0) return false;
+        if (!XF && n == 0) return false;
+        if (n > 0) {
         n -= (n & 1) ^ 1;
         const uint64_t lanes = (~0ull << hl) & (hl + n < 64 ? ~(~0ull << (hl + n)) : ~0ull);
         const float dtl = lg_lane * (__builtin_amdgcn_inverse_ballot_w64(inf) ? F.rtot : B.rtot);
@@ -1166,6 +1181,8 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
         }
         mark(6);
         return true;
+        }
+        // this hop is not the hop back: taken below
       }
     }
     const double target = readlane_f64(tlane, hl);
@@ -1189,7 +1206,13 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
         }
       }
     }
-    if (sel != F.psel) return false;
+    if (sel < 0) return false;   // target rounded onto the total: the general step's fallback
+    // an uncached row: its partner prefix as the general hop forms it
+    const bool pcached = sel == F.psel;
+    if (!pcached) {
+      if constexpr (!XF) return false;
+      else partner_pass(F, sel);
+    }
     int pick = -1;
     if constexpr (C == 1) {
       pick = lowest_bit(F.pnz[0] & ballot(F.pcum[0] > rr));
@@ -1215,7 +1238,7 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
       nocc[c] &= accm[c];
       back = back && nocc[c] == B.key[c];
     }
-    if (!back) return false;
+    if (!XF && !back) return false;
 #pragma unroll
     for (int c = 0; c < C; ++c) occ[c] = nocc[c];
     // electrode tallies on every lane (only the electrode lanes' are kept): the
@@ -1228,6 +1251,19 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
     }
     if (lane == hl) dtv = lg_lane * F.rtot;
     ++h;
+    if constexpr (PH) {
+      ++ph[7];
+      if (pcached) ++ph[8];
+    }
+    if (!back) {
+      // a hop into a new state: taken; the caller evaluates the state into the
+      // other entry (the rank-1 update, where used, takes this hop's ends)
+      psrc = src;
+      pdst = dst;
+      fresh = true;
+      mark(6);
+      return false;
+    }
     if (F.plast == -1) {
       // the first hop back from F since the entries changed: at a high miss
       // rate most pairs of states never come to a leap, so the bounds wait
@@ -1258,10 +1294,6 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
         F.dlt[c] = (s == dst ? 1 : 0) - (s == src ? 1 : 0);
       }
     }
-    if constexpr (PH) {
-      ++ph[7];
-      ++ph[8];
-    }
     mark(6);
     return true;
   };
@@ -1287,6 +1319,13 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
               one = true;
               break;
             }
+        }
+        if (fresh) {
+          // a hop into a new state was taken: the state goes into the other entry
+          fresh = false;
+          cur_ok = false;
+          one = !one;
+          continue;
         }
         if (h >= h_lim) continue;   // the range's end or a special hop
       }
