@@ -9,6 +9,8 @@ lone: the hop latency of one trajectory alone (the long tail of an uneven batch
 fp64phases: shader cycles per hop and phase of the FP64 dense kernel;
 fp32phases: the same for the FP32 batched kernels, on the 4096-trajectory N=30
       ensemble and on lone trajectories (solo and team);
+fp32tail: the ensemble's per-trajectory times, and the phases of its slowest,
+      median and fastest trajectories rerun alone, with the state store on and off;
 chain: latency of dependent FP64 / FP32 adds in one wave.
 """
 from __future__ import annotations
@@ -168,6 +170,55 @@ def fp32_phases_lone(N: int, P: int, form: int, team: int, hops: int = 100_000,
     return r
 
 
+def fp32_tail(K: int = 4096, hops: int = 100_000, group: int = 32) -> list:
+    """The launch's tail: the K-trajectory N=30 ensemble's per-trajectory device
+    times (kmc_batch_timing), then the `group` slowest, median and fastest
+    trajectories rerun as batches of their own (a SIMD each), with the state
+    store on and off: their times and phase cycles per hop."""
+    import numpy as np
+    from . import workloads
+    ens = workloads.ensemble(0, K, hops)
+    net = ens.net
+    lay = engine.Layout(net.N, net.P, net.nu, net.kT, net.I_0, net.R, net.distances,
+                        net.transitions_constant)
+    out = []
+    try:
+        es = lay.energy_sets(net.E_constant, net.site_energies[net.N:])
+        b = engine.Batch(engine.make_trajs(K, lay.id, es[0], ens.seeds, hops, net.occupation))
+        try:
+            b.run(precision=_lib.FP32_BATCHED, waves_per_group=4, table_form=1)
+            ms = b.sync()
+            sec = b.timing()
+        finally:
+            b.close()
+        order = np.argsort(sec)
+        out.append({"workload": f"{K} x {hops} hops, N=30 P=8", "kernel_ms": round(ms, 3),
+                    "traj_ms_p0_50_90_99_100": [round(float(x) * 1e3, 3)
+                                                for x in np.percentile(sec, [0, 50, 90, 99, 100])]})
+        mid = K // 2 - group // 2
+        for name, idx in (("slowest", order[-group:]), ("median", order[mid:mid + group]),
+                          ("fastest", order[:group])):
+            g = engine.Batch(engine.make_trajs(group, lay.id, es[0], ens.seeds[idx], hops,
+                                               net.occupation))
+            try:
+                for store in (0, -1):
+                    run = dict(precision=_lib.FP32_BATCHED, waves_per_group=4, table_form=1,
+                               state_store=store)
+                    g.run(**run)
+                    g.sync()
+                    t = g.timing()   # the ordinary kernel's, not the phase-timing run's
+                    r = _fp32_phase_run(g, run, group * hops)
+                    r.update({"group": f"{group} {name}", "state_store": store,
+                              "traj_ms_mean_max": [round(float(t.mean()) * 1e3, 3),
+                                                   round(float(t.max()) * 1e3, 3)]})
+                    out.append(r)
+            finally:
+                g.close()
+    finally:
+        lay.close()
+    return out
+
+
 CHAIN_KINDS = ["v_add_f64 dependent (64 lanes)", "v_add_f64 dependent (1 lane)",
                "v_add_f32 dependent", "v_fma_f64 dependent", "v_add_f64 4 interleaved chains"]
 
@@ -185,7 +236,8 @@ def chain_latency(iters: int = 4096) -> dict:
 
 def main() -> None:
     ap = argparse.ArgumentParser()
-    ap.add_argument("what", choices=["copy", "lds", "lone", "fp64phases", "fp32phases", "chain"])
+    ap.add_argument("what", choices=["copy", "lds", "lone", "fp64phases", "fp32phases", "fp32tail",
+                                     "chain"])
     ap.add_argument("--bytes", type=int, default=1 << 30)
     ap.add_argument("--repeats", type=int, default=5)
     ap.add_argument("--width", type=int, default=-1, help="4, 8, 16, 0 (stream); -1 all")
@@ -209,6 +261,9 @@ def main() -> None:
                 if team == 1 and form == 3 or team == 4 and N + P <= 64:
                     continue
                 print(json.dumps(fp32_phases_lone(N, P, form, team)), flush=True)
+    elif args.what == "fp32tail":
+        for r in fp32_tail():
+            print(json.dumps(r), flush=True)
     elif args.what == "chain":
         print(json.dumps(chain_latency()))
     else:
