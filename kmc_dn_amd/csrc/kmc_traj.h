@@ -667,6 +667,7 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
   E0.plast = E1.plast = -1;
   bool cur_ok = false;   // the current state's entry holds it
   bool fresh = false;    // a fast hop has just taken a hop into a new state
+  bool noback = false;   // the next hop is known not to be the hop back (a leap's run ended)
   // Energies of a new state.  One-chunk solo code: the recompute's plus the
   // columns of the acceptors changed since, a function of the occupation alone
   // (what lets the state store restore a state's bits).  Teams and kernels
@@ -1149,7 +1150,7 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
     const int hl = h & 63;
     const double tlane = ue_lane * F.tot;   // every lane's hop's target in state F
     {
-      if ((F.plast | B.plast) >= 0) {
+      if ((F.plast | B.plast) >= 0 && !noback) {
         const double tb = ue_lane * B.tot;
         const float rf = (float)(tlane - F.exs), rb = (float)(tb - B.exs);
         const uint64_t okf =
@@ -1164,6 +1165,9 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
         if ((unsigned)n > (unsigned)nmax) n = nmax;
         if (!XF && n == 0) return false;
         if (n > 0) {
+        // a run of odd length that ends inside the range: the hop after it is
+        // not the hop back, and the next fast hop skips this test
+        noback = (n & 1) && n < nmax;
         n -= (n & 1) ^ 1;
         const uint64_t lanes = (~0ull << hl) & (hl + n < 64 ? ~(~0ull << (hl + n)) : ~0ull);
         const float dtl = lg_lane * (__builtin_amdgcn_inverse_ballot_w64(inf) ? F.rtot : B.rtot);
@@ -1185,6 +1189,7 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
         // this hop is not the hop back: taken below
       }
     }
+    noback = false;
     const double target = readlane_f64(tlane, hl);
     // sel and pick are -1 when no lane qualifies (s_ff1 of an empty mask); a
     // readlane from lane -1 reads lane 63 and its value goes unused: psel is
