@@ -471,6 +471,10 @@ __device__ __forceinline__ uint64_t in_vgpr(uint64_t x) {
   asm("" : "+v"(x));
   return x;
 }
+__device__ __forceinline__ int in_vgpr_i(int x) {
+  asm("" : "+v"(x));
+  return x;
+}
 __device__ __forceinline__ float in_vgpr(float x) {
   asm("" : "+v"(x));
   return x;
@@ -668,6 +672,15 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
   bool cur_ok = false;   // the current state's entry holds it
   bool fresh = false;    // a fast hop has just taken a hop into a new state
   bool noback = false;   // the next hop is known not to be the hop back (a leap's run ended)
+  // New states of the current recompute interval, and whether the last interval
+  // had more than 16 (held in VGPRs: touched on new states and at recomputes
+  // only, so they take no scalar registers from the hop loop).  Such a
+  // trajectory hops among three or four states and is among the slowest of a
+  // batch, whose launch lasts as long as they do: its waves issue at priority
+  // 2 instead of 1 outside the hop's chain (graded priority kernels), ahead of
+  // the faster waves that share their SIMDs.  No result depends on it.
+  int nnew = in_vgpr_i(0);
+  int slowv = in_vgpr_i(0);
   // Energies of a new state.  One-chunk solo code: the recompute's plus the
   // columns of the acceptors changed since, a function of the occupation alone
   // (what lets the state store restore a state's bits).  Teams and kernels
@@ -898,7 +911,10 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
     if (pick < 0) pick = last_nz;
     if (pick < 0) return -1;
     mark(4);
-    if constexpr (PRIO) __builtin_amdgcn_s_setprio(1);
+    if constexpr (PRIO) {
+      if (__builtin_amdgcn_readfirstlane(slowv)) __builtin_amdgcn_s_setprio(2);
+      else __builtin_amdgcn_s_setprio(1);
+    }
     const int src = F.udst ? sel : pick;
     const int dst = F.udst ? pick : sel;
     const bool tally = h >= prehops;
@@ -986,6 +1002,12 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
       E0.val = E1.val = 0;
       svalid = 0;
       snext = 0;
+      slowv = in_vgpr_i(nnew > 16 ? 1 : 0);
+      nnew = in_vgpr_i(0);
+      if constexpr (PRIO) {
+        if (__builtin_amdgcn_readfirstlane(slowv)) __builtin_amdgcn_s_setprio(2);
+        else __builtin_amdgcn_s_setprio(1);
+      }
     }
     special = next_special(h, false);
     h_lim = special < h_stop ? special : h_stop;
@@ -1001,6 +1023,7 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
       if (__builtin_expect(h == special, 0) && !special_work()) return false;
     }
     if (!cur_ok) {
+      nnew = in_vgpr_i(nnew + 1);
       // a new state (F's old state, two hops back, leaves the entries)
       B.plast = -1;   // B's hop back led into the state that F held
       bool restored = false;
