@@ -677,8 +677,12 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
   // only, so they take no scalar registers from the hop loop).  Such a
   // trajectory hops among three or four states and is among the slowest of a
   // batch, whose launch lasts as long as they do: its waves issue at priority
-  // 2 instead of 1 outside the hop's chain (graded priority kernels), ahead of
-  // the faster waves that share their SIMDs.  No result depends on it.
+  // 2 instead of 1 outside the hop's chain, ahead of the faster waves that
+  // share their SIMDs.  One-chunk kernels with graded priority only: in an
+  // uneven batch the solo kernel's miss-heavy waves would go ahead of the team
+  // waves that carry its longest trajectories (3.5 % longer).  No result
+  // depends on it.
+  constexpr bool kSlowPrio = PRIO && XF && T == 1;
   int nnew = in_vgpr_i(0);
   int slowv = in_vgpr_i(0);
   // Energies of a new state.  One-chunk solo code: the recompute's plus the
@@ -911,9 +915,11 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
     if (pick < 0) pick = last_nz;
     if (pick < 0) return -1;
     mark(4);
-    if constexpr (PRIO) {
+    if constexpr (kSlowPrio) {
       if (__builtin_amdgcn_readfirstlane(slowv)) __builtin_amdgcn_s_setprio(2);
       else __builtin_amdgcn_s_setprio(1);
+    } else if constexpr (PRIO) {
+      __builtin_amdgcn_s_setprio(1);
     }
     const int src = F.udst ? sel : pick;
     const int dst = F.udst ? pick : sel;
@@ -1002,9 +1008,9 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
       E0.val = E1.val = 0;
       svalid = 0;
       snext = 0;
-      slowv = in_vgpr_i(nnew > 16 ? 1 : 0);
-      nnew = in_vgpr_i(0);
-      if constexpr (PRIO) {
+      if constexpr (kSlowPrio) {
+        slowv = in_vgpr_i(nnew > 16 ? 1 : 0);
+        nnew = in_vgpr_i(0);
         if (__builtin_amdgcn_readfirstlane(slowv)) __builtin_amdgcn_s_setprio(2);
         else __builtin_amdgcn_s_setprio(1);
       }
@@ -1023,7 +1029,7 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
       if (__builtin_expect(h == special, 0) && !special_work()) return false;
     }
     if (!cur_ok) {
-      nnew = in_vgpr_i(nnew + 1);
+      if constexpr (kSlowPrio) nnew = in_vgpr_i(nnew + 1);
       // a new state (F's old state, two hops back, leaves the entries)
       B.plast = -1;   // B's hop back led into the state that F held
       bool restored = false;
