@@ -132,3 +132,20 @@ def test_phase_probe_counts_restores_only_with_the_store(gpu_engine):
         b.close()
     finally:
         lay.close()
+
+
+def test_store_option_is_validated(gpu_engine):
+    net, lay, es = _c1_layout()
+    try:
+        b = engine.Batch(engine.make_trajs(8, lay.id, es, np.arange(8, dtype=np.uint64), 1000,
+                                           net.occupation))
+        for bad in (-2, 5):
+            with pytest.raises(_lib.KmcError) as e:
+                b.run(precision=_lib.FP32_BATCHED, state_store=bad)
+            assert e.value.code == _lib.ERR_ARGS
+        b.run(precision=_lib.FP32_BATCHED, state_store=3)   # at most 3 slots: accepted
+        b.sync()
+        assert (b.results().status == 0).all()
+        b.close()
+    finally:
+        lay.close()
