@@ -416,3 +416,33 @@ def test_electrode_closed_form_and_member_fallback(gpu_engine):
         pos = out[3].currents(P)[sl]
         for form in (1, 2):
             _compare(out[form].currents(P)[sl], pos, f"{half}: table {form} vs positions")
+
+
+@pytest.mark.parametrize("N,P,form", [(30, 8, 1), (30, 8, 3), (50, 8, 2), (90, 10, 2)])
+def test_fast_hops_and_leaps_equal_general_steps(gpu_engine, N, P, form):
+    """Kernels that record take the general step on every hop; the others take
+    fast hops (with inline partner passes and hops into new states in one-chunk
+    kernels) and leaps over runs of hops back.  Both give the same trajectories
+    bit for bit: clock, tallies and occupation (S = 38 and 58 one-chunk, S = 100
+    two chunks)."""
+    net = random_network(40 + N, N, P, I_0=50.0, a=0.1, vmax=1.0)
+    lay = engine.Layout(N, P, net["nu"], net["kT"], net["I_0"], net["R"], net["distances"],
+                        net["transitions_constant"])
+    lay.set_positions(0.1 * net["R"], _positions(net))
+    es = lay.energy_sets(net["E_constant"], net["site_energies"][N:])
+    K, hops = 256, 20_000
+    b = engine.Batch(engine.make_trajs(K, lay.id, es[0], np.arange(K, dtype=np.uint64), hops,
+                                       net["occupation"], prehops=1000))
+    print(f"[gpu] fast path vs general steps: S={N + P} form {form}", flush=True)
+    out = []
+    for rec in (False, True):
+        b.run(precision=_lib.FP32_BATCHED, table_form=form, record=rec)
+        b.sync()
+        r = b.results()
+        assert (r.status == 0).all()
+        out.append(r)
+    b.close()
+    lay.close()
+    assert np.array_equal(out[0].time, out[1].time)
+    assert np.array_equal(out[0].eo, out[1].eo)
+    assert np.array_equal(out[0].occ_bits, out[1].occ_bits)
