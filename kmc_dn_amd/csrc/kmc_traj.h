@@ -516,11 +516,15 @@ __device__ __forceinline__ uint64_t uniform_u64(uint64_t x) {
 // store changes no result, only how often a state is evaluated.  Trajectories
 // that hop among three or four states (at low bias, the slowest of an
 // ensemble) miss the two entries on several per cent of their hops.
-// XF: fast hops also take the hops into an uncached row or a new state (one-chunk
-// kernels; kernels over several site chunks keep the shorter fast hop for their
-// one-chunk code too, which holds their registers to the occupancy they run at).
+// XF: fast hops also take the hops into an uncached row or a new state, and skip
+// the leap test where a run's end has settled it (one-chunk kernels with graded
+// priority, whose launches end with their slowest trajectories; kernels over
+// several site chunks keep the shorter fast hop for their one-chunk code too,
+// which holds their registers to the occupancy they run at, and the variant
+// without priorities for launches that queue behind full SIMDs, whose
+// throughput the longer code cost 5 %).
 template <int C, bool REC, class Pol, int T = 1, bool PRIO = true, bool PH = false,
-          bool XF = (C == 1)>
+          bool XF = (C == 1) && PRIO>
 __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, const Pol& pol,
                                int lane, const Team<T, C>& team = Team<T, C>{},
                                unsigned char* store = nullptr) {
@@ -1179,7 +1183,7 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
     const int hl = h & 63;
     const double tlane = ue_lane * F.tot;   // every lane's hop's target in state F
     {
-      if ((F.plast | B.plast) >= 0 && !noback) {
+      if ((F.plast | B.plast) >= 0 && !(XF && noback)) {
         const double tb = ue_lane * B.tot;
         const float rf = (float)(tlane - F.exs), rb = (float)(tb - B.exs);
         const uint64_t okf =
@@ -1196,7 +1200,7 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
         if (n > 0) {
         // a run of odd length that ends inside the range: the hop after it is
         // not the hop back, and the next fast hop skips this test
-        noback = (n & 1) && n < nmax;
+        if constexpr (XF) noback = (n & 1) && n < nmax;
         n -= (n & 1) ^ 1;
         const uint64_t lanes = (~0ull << hl) & (hl + n < 64 ? ~(~0ull << (hl + n)) : ~0ull);
         const float dtl = lg_lane * (__builtin_amdgcn_inverse_ballot_w64(inf) ? F.rtot : B.rtot);
@@ -1218,7 +1222,7 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
         // this hop is not the hop back: taken below
       }
     }
-    noback = false;
+    if constexpr (XF) noback = false;
     const double target = readlane_f64(tlane, hl);
     // sel and pick are -1 when no lane qualifies (s_ff1 of an empty mask); a
     // readlane from lane -1 reads lane 63 and its value goes unused: psel is
