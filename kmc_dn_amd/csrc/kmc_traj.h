@@ -516,15 +516,10 @@ __device__ __forceinline__ uint64_t uniform_u64(uint64_t x) {
 // store changes no result, only how often a state is evaluated.  Trajectories
 // that hop among three or four states (at low bias, the slowest of an
 // ensemble) miss the two entries on several per cent of their hops.
-// XF: fast hops also take the hops into an uncached row or a new state, and skip
-// the leap test where a run's end has settled it (one-chunk kernels with graded
-// priority, whose launches end with their slowest trajectories; kernels over
-// several site chunks keep the shorter fast hop for their one-chunk code too,
-// which holds their registers to the occupancy they run at, and the variant
-// without priorities for launches that queue behind full SIMDs, whose
-// throughput the longer code cost 5 %).
+// ONE: the code runs in a kernel built for one site chunk (false for the
+// one-chunk code of kernels over several chunks).
 template <int C, bool REC, class Pol, int T = 1, bool PRIO = true, bool PH = false,
-          bool XF = (C == 1) && PRIO>
+          bool ONE = (C == 1)>
 __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, const Pol& pol,
                                int lane, const Team<T, C>& team = Team<T, C>{},
                                unsigned char* store = nullptr) {
@@ -674,8 +669,6 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
   E0.val = E1.val = 0;
   E0.plast = E1.plast = -1;
   bool cur_ok = false;   // the current state's entry holds it
-  bool fresh = false;    // a fast hop has just taken a hop into a new state
-  bool noback = false;   // the next hop is known not to be the hop back (a leap's run ended)
   // New states of the current recompute interval, and whether the last interval
   // had more than 16 (held in VGPRs: touched on new states and at recomputes
   // only, so they take no scalar registers from the hop loop).  Such a
@@ -686,7 +679,7 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
   // uneven batch the solo kernel's miss-heavy waves would go ahead of the team
   // waves that carry its longest trajectories (3.5 % longer).  No result
   // depends on it.
-  constexpr bool kSlowPrio = PRIO && XF && T == 1;
+  constexpr bool kSlowPrio = PRIO && ONE && T == 1;
   int nnew = in_vgpr_i(0);
   int slowv = in_vgpr_i(0);
   // Energies of a new state.  One-chunk solo code: the recompute's plus the
@@ -1154,18 +1147,17 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
     mark(5);
     return h < h_stop;
   };
-  // A hop from a cached state in its shortest form: the current state's entry
-  // F is valid.  The arithmetic is the general hop's (same target, remainder,
-  // partner prefix, partner and clock bits); only the bookkeeping that such a
-  // hop cannot need is left out, so the scalar unit that the waves of a CU
-  // share issues a third of the general path's instructions.  A row whose
-  // partner prefix is not cached gets it here; a hop back into the state of the
-  // other entry B returns true, a hop into a new state is taken and returns
-  // false with `fresh` set (the caller evaluates the state).  The rare rest (a
-  // target rounded onto the total, a row without a partner, the end of the
+  // A hit hop in its shortest form: the current state's entry F is valid, its
+  // partner prefix was formed for the row the event falls in and the hop leads
+  // back into the state of the other entry B.  The arithmetic is the general
+  // hop's (same target, remainder, partner and clock bits); only the
+  // bookkeeping that such a hop cannot need is left out, so the scalar unit
+  // that the waves of a CU share issues a third of the general path's
+  // instructions.  Anything else (an uncached row, a new state, the end of the
   // range, uniforms or a recompute due) returns false with nothing changed, and
-  // the general step takes the hop.  Kernels that record take the general step
-  // on every hop.
+  // the general step takes the hop.  (Taking the hops into an uncached row or a
+  // new state here too made the common path's code longer: 7 % slower.)
+  // Kernels that record take the general step on every hop.
   //
   // Leaps: once the hop back is known from both states
   // (plast), whether hop k of the current uniform block takes it depends only
@@ -1183,7 +1175,7 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
     const int hl = h & 63;
     const double tlane = ue_lane * F.tot;   // every lane's hop's target in state F
     {
-      if ((F.plast | B.plast) >= 0 && !(XF && noback)) {
+      if ((F.plast | B.plast) >= 0) {
         const double tb = ue_lane * B.tot;
         const float rf = (float)(tlane - F.exs), rb = (float)(tb - B.exs);
         const uint64_t okf =
@@ -1196,11 +1188,10 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
         int n = lowest_bit(~run);
         const int nmax = 64 - hl < h_lim - h ? 64 - hl : h_lim - h;
         if ((unsigned)n > (unsigned)nmax) n = nmax;
-        if (!XF && n == 0) return false;
+        if (n == 0) return false;
+        // the run applied in a block of its own (n > 0 always holds here): the
+        // register allocator then spills less in the hop loop (60 against 92 B)
         if (n > 0) {
-        // a run of odd length that ends inside the range: the hop after it is
-        // not the hop back, and the next fast hop skips this test
-        if constexpr (XF) noback = (n & 1) && n < nmax;
         n -= (n & 1) ^ 1;
         const uint64_t lanes = (~0ull << hl) & (hl + n < 64 ? ~(~0ull << (hl + n)) : ~0ull);
         const float dtl = lg_lane * (__builtin_amdgcn_inverse_ballot_w64(inf) ? F.rtot : B.rtot);
@@ -1219,10 +1210,8 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
         mark(6);
         return true;
         }
-        // this hop is not the hop back: taken below
       }
     }
-    if constexpr (XF) noback = false;
     const double target = readlane_f64(tlane, hl);
     // sel and pick are -1 when no lane qualifies (s_ff1 of an empty mask); a
     // readlane from lane -1 reads lane 63 and its value goes unused: psel is
@@ -1245,12 +1234,7 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
       }
     }
     if (sel < 0) return false;   // target rounded onto the total: the general step's fallback
-    // an uncached row: its partner prefix as the general hop forms it
-    const bool pcached = sel == F.psel;
-    if (!pcached) {
-      if constexpr (!XF) return false;
-      else partner_pass(F, sel);
-    }
+    if (sel != F.psel) return false;
     int pick = -1;
     if constexpr (C == 1) {
       pick = lowest_bit(F.pnz[0] & ballot(F.pcum[0] > rr));
@@ -1276,7 +1260,7 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
       nocc[c] &= accm[c];
       back = back && nocc[c] == B.key[c];
     }
-    if (!XF && !back) return false;
+    if (!back) return false;
 #pragma unroll
     for (int c = 0; c < C; ++c) occ[c] = nocc[c];
     // electrode tallies on every lane (only the electrode lanes' are kept): the
@@ -1291,16 +1275,7 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
     ++h;
     if constexpr (PH) {
       ++ph[7];
-      if (pcached) ++ph[8];
-    }
-    if (!back) {
-      // a hop into a new state: taken; the caller evaluates the state into the
-      // other entry (the rank-1 update, where used, takes this hop's ends)
-      psrc = src;
-      pdst = dst;
-      fresh = true;
-      mark(6);
-      return false;
+      ++ph[8];
     }
     if (F.plast == -1) {
       // the first hop back from F since the entries changed: at a high miss
@@ -1357,13 +1332,6 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
               one = true;
               break;
             }
-        }
-        if (fresh) {
-          // a hop into a new state was taken: the state goes into the other entry
-          fresh = false;
-          cur_ok = false;
-          one = !one;
-          continue;
         }
         if (h >= h_lim) continue;   // the range's end or a special hop
       }
