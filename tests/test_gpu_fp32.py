@@ -421,8 +421,7 @@ def test_electrode_closed_form_and_member_fallback(gpu_engine):
 @pytest.mark.parametrize("N,P,form", [(30, 8, 1), (30, 8, 3), (50, 8, 2), (90, 10, 2)])
 def test_fast_hops_and_leaps_equal_general_steps(gpu_engine, N, P, form):
     """Kernels that record take the general step on every hop; the others take
-    fast hops (with inline partner passes and hops into new states in one-chunk
-    kernels) and leaps over runs of hops back.  Both give the same trajectories
+    fast hops and leaps over runs of hops back.  Both give the same trajectories
     bit for bit: clock, tallies and occupation (S = 38 and 58 one-chunk, S = 100
     two chunks)."""
     net = random_network(40 + N, N, P, I_0=50.0, a=0.1, vmax=1.0)
