@@ -708,7 +708,7 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
     double* inc = slot_inc(v);
     inc[lane] = W.inc[0];
     inc[64 + lane] = W.exc[0];
-    reinterpret_cast<float*>(inc + 128)[lane] = W.se[0];
+    inc[128 + lane] = (double)W.se[0];   // (exact both ways)
     if (lane == 0) {
       skey[v] = W.key[0];
       snzr[v] = W.nzr[0];
@@ -1045,7 +1045,7 @@ __device__ void run_trajectory(const BatchParams& p, int t, const LayoutDev& L, 
             const int v = __builtin_ctz(found);
             double* inc = slot_inc(v);
             const double si = inc[lane], sx = inc[64 + lane];
-            const float sv = reinterpret_cast<const float*>(inc + 128)[lane];
+            const float sv = (float)inc[128 + lane];
             const uint64_t nz = snzr[v];
             if (F.val) store_put(v, F);
             else svalid &= ~(1u << v);

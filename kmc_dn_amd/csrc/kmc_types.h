@@ -134,11 +134,12 @@ struct BatchParams {
 // The state store of a one-chunk solo trajectory: states evicted from its two
 // register entries, kept in LDS per wave.  Layout: STORE_MAX_SLOTS occupation
 // keys (u64), then STORE_MAX_SLOTS nonzero-row lane masks (u64), then per slot
-// the FP64 inclusive and exclusive row prefixes and the FP32 energies of the
-// 64 lanes.
+// the FP64 inclusive and exclusive row prefixes and the energies (FP32 values
+// held as FP64, so that every array of a slot has the same per-lane stride) of
+// the 64 lanes.
 constexpr int STORE_MAX_SLOTS = 4;
 constexpr int STORE_HEADER_BYTES = 2 * STORE_MAX_SLOTS * 8;
-constexpr int STORE_SLOT_BYTES = 64 * (8 + 8 + 4);
+constexpr int STORE_SLOT_BYTES = 64 * (8 + 8 + 8);
 constexpr int store_wave_bytes(int slots) {
   return slots > 0 ? STORE_HEADER_BYTES + slots * STORE_SLOT_BYTES : 0;
 }
