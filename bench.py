@@ -37,18 +37,18 @@ METRIC = "hops/sec, 4096-traj N=30 ensemble vs plain; p50/p99 latency of served 
 # divides by the LDS read peak measured in the same run (ds_read_b128 stream probe).
 LDS_PEAK_SPEC_GBS = 256 * 256 * 2.4
 # HBM bytes per launch of the batched kernel on the default workload (4096 x 1e5 hops,
-# N=30/P=8, auto table form and workgroup size): rocprofv3 --pmc FETCH_SIZE 5738.75 KiB
-# and WRITE_SIZE 14976 KiB per dispatch, scaled by the copy-kernel calibration factors
-# 2.0 (FETCH) and 1.0 (WRITE): (2 * 5738.75 + 14976) * 1024 bytes -- the launch's own
+# N=30/P=8, auto table form and workgroup size): rocprofv3 --pmc FETCH_SIZE 5738.81 KiB
+# and WRITE_SIZE 17024 KiB per dispatch, scaled by the copy-kernel calibration factors
+# 2.0 (FETCH) and 1.0 (WRITE): (2 * 5738.81 + 17024) * 1024 bytes -- the launch's own
 # I/O (1.1 MB) and register-spill scratch (the one-chunk kernels are held to 128 VGPRs).
-HBM_TRAFFIC_BYTES = 27_088_384
+HBM_TRAFFIC_BYTES = 29_185_659
 # VALU and scalar instructions per hop and wave of the batched kernel on the default
-# workload (rocprofv3 --pmc SQ_INSTS_VALU 2.560e9, SQ_INSTS_SALU 2.356e9 per dispatch of
+# workload (rocprofv3 --pmc SQ_INSTS_VALU 2.561e9, SQ_INSTS_SALU 2.362e9 per dispatch of
 # 4.096e8 hops), and the issue peak of each: one wave64 VALU instruction per 4 cycles on
 # each of a CU's 4 SIMDs, and one scalar instruction per cycle from the CU's shared
 # scalar unit, i.e. one per CU and cycle each, 256 CUs at 2.4 GHz.
 VALU_INSTS_PER_HOP = 6.25
-SALU_INSTS_PER_HOP = 5.75
+SALU_INSTS_PER_HOP = 5.77
 ISSUE_PEAK = 256 * 2.4e9
 
 
@@ -439,7 +439,7 @@ def main() -> int:
                     "hops between the two cached states a whole uniform block at a time.  The "
                     "launch ends with its slowest trajectories (those that hop among three or "
                     "four states), bound by the latency of their hop chains (LdsUtil 3%, "
-                    "VALUBusy 34%, waves waiting on dependencies 42% of their cycles); "
+                    "VALUBusy 34%, waves waiting on dependencies 41% of their cycles); "
                     "valu/salu_issue_frac = profiled instructions per hop (6.25 VALU, 5.75 "
                     "scalar) x hops / kernel time / (one per CU and cycle)",
         },
