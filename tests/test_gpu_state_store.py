@@ -6,9 +6,10 @@ recomputes the energies of a state are a function of its occupation alone (the
 recompute's energies plus the Coulomb columns of the acceptors changed since,
 in ascending site order), so the restored entry holds the bits an evaluation
 gives: runs with the store off (state_store = -1), with 1 or 2 slots and with
-the automatic size (0) agree bit for bit, in every pair-term form, with
-records, and for one-chunk networks inside a three-chunk positions kernel.  The
-phase probe counts restores only with the store on.  No reference counterpart
+the automatic size (0) agree bit for bit, in every pair-term form and with
+records; one-chunk networks run inside a three-chunk positions kernel (whose
+one-chunk code takes no store) equal the same trajectories in a one-chunk batch
+of their own.  The phase probe counts restores only with the store on.  No reference counterpart
 (the Go library's occupation cache, simulation.go:254-296, keeps every state).
 """
 import ctypes as C
@@ -75,9 +76,10 @@ def test_store_changes_no_bits(gpu_engine, form):
 
 def test_store_in_one_chunk_code_of_a_three_chunk_kernel(gpu_engine):
     """A batch with an S = 140 network runs the positions kernel of three site
-    chunks; its S = 38 trajectories take the one-chunk code with the store, and
-    equal, bit for bit, the same trajectories without the store and in a batch of
-    their own."""
+    chunks; its S = 38 trajectories take that kernel's one-chunk code, which the
+    host gives no store (state_store 0 and -1 run alike there), and equal, bit for
+    bit, the same trajectories in a batch of their own, which runs the one-chunk
+    positions kernel with the store the host sizes for it."""
     small = random_network(5, 30, 8, I_0=50.0, a=0.1, vmax=1.0)
     big = random_network(6, 128, 12, I_0=50.0, a=0.1, vmax=1.0)
     lays, trajs = [], []
@@ -94,7 +96,7 @@ def test_store_in_one_chunk_code_of_a_three_chunk_kernel(gpu_engine):
         print("[gpu] state store in a three-chunk positions kernel", flush=True)
         mixed = engine.Batch(np.concatenate(trajs))
         outs = []
-        for store in (-1, 0):
+        for store in (-1, 0):   # the option is accepted; no slots in this kernel
             mixed.run(precision=_lib.FP32_BATCHED, table_form=3, state_store=store)
             mixed.sync()
             outs.append(_bits(mixed.results()))
