@@ -74,7 +74,7 @@ def test_store_changes_no_bits(gpu_engine, form):
         lay.close()
 
 
-def test_store_in_one_chunk_code_of_a_three_chunk_kernel(gpu_engine):
+def test_one_chunk_code_of_a_three_chunk_kernel_equals_a_one_chunk_batch(gpu_engine):
     """A batch with an S = 140 network runs the positions kernel of three site
     chunks; its S = 38 trajectories take that kernel's one-chunk code, which the
     host gives no store (state_store 0 and -1 run alike there), and equal, bit for
@@ -93,7 +93,7 @@ def test_store_in_one_chunk_code_of_a_three_chunk_kernel(gpu_engine):
         trajs.append(engine.make_trajs(K, lay.id, es[0], np.arange(K, dtype=np.uint64), 10_000,
                                        net["occupation"]))
     try:
-        print("[gpu] state store in a three-chunk positions kernel", flush=True)
+        print("[gpu] one-chunk trajectories in a three-chunk positions kernel vs alone", flush=True)
         mixed = engine.Batch(np.concatenate(trajs))
         outs = []
         for store in (-1, 0):   # the option is accepted; no slots in this kernel
